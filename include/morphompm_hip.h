@@ -1,0 +1,147 @@
+/*
+ * morphompm_hip.h -- C ABI of the MI355X (gfx950) MLS-MPM growth step and its
+ * hand-written adjoint.
+ *
+ * One shared library (morphompm_amd/_lib/libmorphompm_hip.so) exports these
+ * symbols.  Conventions:
+ *   - every buffer is a caller-owned device pointer; the library never
+ *     allocates device memory;
+ *   - calls are stream-ordered and asynchronous on `stream` (a hipStream_t,
+ *     passed as void*; NULL is the default stream);
+ *   - functions return MPM_OK (0) or a negative MPM_E* code and never throw or
+ *     abort across the ABI; mpm_last_error() holds a per-thread message;
+ *   - _f32 / _f64 pairs share one templated implementation.
+ *
+ * Particle layout (structure of arrays, row stride `n_slots`):
+ *   dyn  [24][n_slots]: rows 0-2 x, 3-5 v, 6-14 C, 15-23 F (3x3 row-major)
+ *   Fg   [9][n_slots] : the growth tensor G; the step uses
+ *                       Fg = I + a (G - I) with a = sched[instance]
+ *                       (sched == NULL means a = 1, i.e. Fg = G exactly)
+ * Grid layout: grid [4][n_nodes]: row 0 node mass, rows 1-3 node velocity.
+ * Sort record for one step: sortbuf int32 [2 n_slots + n_bins + 1]:
+ *   rank[n_slots] | sorted[n_slots] | bin_start[n_bins + 1]
+ *
+ * Reference interfaces these replace (paths in 2nugu/morphompm):
+ *   python/morphompm/constitutive.py:1-12 (stress / stress_vjp seam),
+ *   python/morphompm/transfer.py:33 (step), :82 (step_vjp).
+ */
+#ifndef MORPHOMPM_HIP_H
+#define MORPHOMPM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_ABI_VERSION 1
+#define MPM_SLOT_ALIGN 256 /* particle and node ranges of an instance start at multiples of this */
+
+enum { MPM_OK = 0, MPM_EINVAL = -1, MPM_ELAUNCH = -2, MPM_ENOTSUP = -3 };
+enum { MPM_NEO_HOOKEAN = 0, MPM_HENCKY = 1, MPM_HERSCHEL_BULKLEY = 2 };
+
+/* One independent simulation (python/morphompm/config.py:21-40 SimConfig plus
+ * its constitutive model).  Node grid nx*ny*nz covers [0, n*dx) per axis;
+ * bins are (nx+2)(ny+2)(nz+2) particle cells keyed by base+2, plus one dead
+ * bin for particles with no in-grid node. */
+typedef struct mpm_instance {
+  int64_t p_off; /* first particle slot, multiple of MPM_SLOT_ALIGN */
+  int64_t g_off; /* first node, multiple of MPM_SLOT_ALIGN */
+  int64_t c_off; /* first bin */
+  int64_t t_off; /* first P2G tile */
+  int32_t n;     /* live particles */
+  int32_t nx, ny, nz;
+  int32_t tx, ty, tz; /* P2G tiles per axis */
+  int32_t model;      /* MPM_NEO_HOOKEAN | MPM_HENCKY | MPM_HERSCHEL_BULKLEY */
+  double dx, dt, damping, pmass, vol0;
+  double prm[6]; /* NH/Hencky: mu, lambda; HB: kappa, K, n, tau_y, m, eps */
+} mpm_instance;
+
+/* A batch of instances laid out for one launch per kernel.  The struct lives
+ * in host memory; the pointers in it are device pointers. */
+typedef struct mpm_batch {
+  int32_t n_inst;
+  int32_t tile[3];   /* P2G tile shape in bins (x, y, z) */
+  int64_t n_slots;   /* particle slots incl. padding, multiple of MPM_SLOT_ALIGN */
+  int64_t n_nodes;   /* node slots incl. padding, multiple of MPM_SLOT_ALIGN */
+  int64_t n_bins;    /* bins of all instances */
+  int64_t n_tiles;   /* P2G tiles of all instances */
+  int32_t block;     /* threads per block of the particle kernels: 64, 128 or 256 */
+  int32_t reserved;
+  const mpm_instance* inst;  /* device [n_inst] */
+  const int32_t* slot_inst;  /* device [n_slots / 256]: instance owning each 256 slots */
+  const int32_t* node_inst;  /* device [n_nodes / 256] */
+  const int32_t* tile_inst;  /* device [n_tiles] */
+} mpm_batch;
+
+int32_t mpm_abi_version(void);
+const char* mpm_last_error(void);
+
+/* Bytes of scratch `ws` the step functions need for this batch and dtype
+ * (0 = f32, 1 = f64).  Negative on error. */
+int64_t mpm_workspace_bytes(const mpm_batch* b, int32_t dtype);
+
+/* Tile shapes this build instantiates, as x,y,z triples; returns the count. */
+int32_t mpm_tile_shapes(int32_t* out_xyz, int32_t max_shapes);
+
+/* Constitutive seam over n particles (SoA [9][stride] per 3x3 field).
+ * Replaces Model.stress (python/morphompm/constitutive.py:88-89,126-132). */
+int32_t mpm_stress_f32(int32_t model, const double* prm6, int64_t n, int64_t stride,
+                       const float* F, const float* Fg, const float* C, float* tau, void* stream);
+int32_t mpm_stress_f64(int32_t model, const double* prm6, int64_t n, int64_t stride,
+                       const double* F, const double* Fg, const double* C, double* tau,
+                       void* stream);
+/* Replaces Model.stress_vjp (python/morphompm/constitutive.py:91-93,134-147).
+ * theta_bar ([2][stride], may be NULL) receives the Herschel-Bulkley
+ * (tau_y, K) cotangents, an extension the reference does not return. */
+int32_t mpm_stress_vjp_f32(int32_t model, const double* prm6, int64_t n, int64_t stride,
+                           const float* F, const float* Fg, const float* C, const float* tau_bar,
+                           float* F_bar, float* Fg_bar, float* C_bar, float* theta_bar,
+                           void* stream);
+int32_t mpm_stress_vjp_f64(int32_t model, const double* prm6, int64_t n, int64_t stride,
+                           const double* F, const double* Fg, const double* C,
+                           const double* tau_bar, double* F_bar, double* Fg_bar, double* C_bar,
+                           double* theta_bar, void* stream);
+
+/* Stable counting sort of particles by base cell (bin), fixed order inside a
+ * bin (ascending slot).  Writes sortbuf.  No reference counterpart: the
+ * reference loops particles in index order (python/morphompm/transfer.py:42). */
+int32_t mpm_sort_by_cell_f32(const mpm_batch* b, const float* dyn, int32_t* sortbuf, void* ws,
+                             void* stream);
+int32_t mpm_sort_by_cell_f64(const mpm_batch* b, const double* dyn, int32_t* sortbuf, void* ws,
+                             void* stream);
+
+/* One forward step: sort, P2G (deterministic, no float atomics), grid update,
+ * G2P.  Replaces transfer.step (python/morphompm/transfer.py:33-79); writes
+ * dyn_out, grid and sortbuf (the latter two are what step_bwd needs). */
+int32_t mpm_step_fwd_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
+                         const float* sched, float* dyn_out, float* grid, int32_t* sortbuf,
+                         int32_t advect, void* ws, void* stream);
+int32_t mpm_step_fwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
+                         const double* sched, double* dyn_out, double* grid, int32_t* sortbuf,
+                         int32_t advect, void* ws, void* stream);
+
+/* Reverse of one step.  Replaces transfer.step_vjp
+ * (python/morphompm/transfer.py:82-158).  Inputs are the forward step's
+ * dyn_in, Fg, sched, dyn_out, grid and sortbuf, and the cotangents of the
+ * step's outputs bar_out [24][n_slots].  Writes bar_in [24][n_slots] and adds
+ * a * (local Fg cotangent) into Fg_bar [9][n_slots] (the reference's
+ * pass-through Fg_b = Fgn_b + local when a = 1).  theta_acc (may be NULL)
+ * accumulates the Herschel-Bulkley (tau_y, K) cotangents per 256-slot block,
+ * [n_slots/256][2] float64. */
+int32_t mpm_step_bwd_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
+                         const float* sched, const float* dyn_out, const float* grid,
+                         const int32_t* sortbuf, const float* bar_out, float* bar_in,
+                         float* Fg_bar, double* theta_acc, int32_t advect, void* ws,
+                         void* stream);
+int32_t mpm_step_bwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
+                         const double* sched, const double* dyn_out, const double* grid,
+                         const int32_t* sortbuf, const double* bar_out, double* bar_in,
+                         double* Fg_bar, double* theta_acc, int32_t advect, void* ws,
+                         void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MORPHOMPM_HIP_H */

@@ -1,0 +1,999 @@
+// MLS-MPM growth step and its hand-written adjoint for gfx950 (MI355X).
+//
+// Per forward step (transfer.step, python/morphompm/transfer.py:33-79):
+//   sort     : bin key per particle -> counting sort (integer atomics only for
+//              the counts) -> per-bin sort by slot, so the order inside every
+//              bin is fixed;
+//   prep     : stress, grown volume and the APIC payload of each particle,
+//              written in sorted order;
+//   p2g tile : one thread per bin sums its particles' 27 stencil contributions
+//              in registers, then adds them into an LDS node tile in 27
+//              barrier-separated rounds (one writer per node per round): a
+//              cell-binned segmented reduction with no float atomics;
+//   grid     : each node sums the <= 8 tile partials covering it in a fixed
+//              order, then applies v = (1 - damping) mom / m;
+//   g2p      : 27-node gather, new v, C, x, F = (I + dt C) F.
+// Per reverse step (transfer.step_vjp, python/morphompm/transfer.py:82-158):
+//   prep_adj : G2P-adjoint payload (nv_bar, C_bar') in sorted order;
+//   p2g tile : the G2P-adjoint scatter as the same segmented reduction;
+//   grid_adj : gmom_bar = s gv_bar / m, gm_bar = -(gv_bar . gv) / m;
+//   p2g_adj  : 27-node gather of (gmom_bar, gm_bar, gv), advection terms, and
+//              the constitutive VJP.
+// Every sum has a fixed order, so results are bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/morphompm_hip.h"
+#include "mpm_math.h"
+
+using namespace mpm;
+
+namespace {
+
+constexpr int kChunk = MPM_SLOT_ALIGN;  // slots per instance-table entry
+constexpr int kPayload = 15;            // fx[3], Q[3], A'[9]
+constexpr int kScanBlock = 256;
+constexpr int kScanPer = 4;             // elements per thread in the bin scan
+
+thread_local char g_err[512] = "";
+
+int set_err(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(MPM_ELAUNCH, "%s: %s", what, hipGetErrorString(e));
+  return MPM_OK;
+}
+
+// Per-instance constants in working precision.
+template <typename T>
+struct Inst {
+  int64_t p_off, g_off, c_off, t_off;
+  int n, nx, ny, nz, tx, ty, tz;
+  T dx, inv_dx, dt, s, pmass, vol0, k4;
+  ModelParams<T> mp;
+};
+
+template <typename T>
+__device__ __forceinline__ Inst<T> load_inst(const mpm_instance* inst, int i) {
+  const mpm_instance& I = inst[i];
+  Inst<T> r;
+  r.p_off = I.p_off; r.g_off = I.g_off; r.c_off = I.c_off; r.t_off = I.t_off;
+  r.n = I.n; r.nx = I.nx; r.ny = I.ny; r.nz = I.nz; r.tx = I.tx; r.ty = I.ty; r.tz = I.tz;
+  r.dx = (T)I.dx;
+  r.inv_dx = (T)(1.0 / I.dx);
+  r.dt = (T)I.dt;
+  r.s = (T)(1.0 - I.damping);
+  r.pmass = (T)I.pmass;
+  r.vol0 = (T)I.vol0;
+  r.k4 = (T)(4.0 / (I.dx * I.dx));
+  r.mp.kind = I.model;
+  for (int k = 0; k < 6; ++k) r.mp.p[k] = (T)I.prm[k];
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void ld3(const T* base, int64_t stride, int64_t p, int row, T v[3]) {
+  for (int k = 0; k < 3; ++k) v[k] = base[(int64_t)(row + k) * stride + p];
+}
+template <typename T>
+__device__ __forceinline__ M3<T> ld9(const T* base, int64_t stride, int64_t p, int row) {
+  M3<T> m;
+  for (int k = 0; k < 9; ++k) m.a[k] = base[(int64_t)(row + k) * stride + p];
+  return m;
+}
+template <typename T>
+__device__ __forceinline__ void st3(T* base, int64_t stride, int64_t p, int row, const T v[3]) {
+  for (int k = 0; k < 3; ++k) base[(int64_t)(row + k) * stride + p] = v[k];
+}
+template <typename T>
+__device__ __forceinline__ void st9(T* base, int64_t stride, int64_t p, int row, const M3<T>& m) {
+  for (int k = 0; k < 9; ++k) base[(int64_t)(row + k) * stride + p] = m.a[k];
+}
+
+// Growth tensor of this step: Fg = I + a (G - I); a == 1 keeps G exactly.
+template <typename T>
+__device__ __forceinline__ M3<T> grown(const M3<T>& G, T a) {
+  if (a == T(1)) return G;
+  M3<T> Fg;
+  for (int k = 0; k < 9; ++k) Fg.a[k] = a * (G.a[k] - ((k % 4 == 0) ? T(1) : T(0)));
+  Fg.a[0] += T(1); Fg.a[4] += T(1); Fg.a[8] += T(1);
+  return Fg;
+}
+
+template <typename T>
+__device__ __forceinline__ void stencil_base(const T x[3], T inv_dx, int base[3], T fx[3]) {
+  for (int a = 0; a < 3; ++a) base[a] = cell_base(x[a], inv_dx, fx[a]);
+}
+
+// dyn row offsets
+constexpr int RX = 0, RV = 3, RC = 6, RF = 15;
+
+// ---------------------------------------------------------------------------
+// sort
+// ---------------------------------------------------------------------------
+__global__ void k_zero_i32(int32_t* a, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sort_key(const mpm_instance* inst, const int32_t* slot_inst, int64_t n_slots,
+                           const T* dyn, int32_t* key, int32_t* slot, int32_t* count) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_slots) return;
+  const Inst<T> I = load_inst<T>(inst, slot_inst[p / kChunk]);
+  if (p - I.p_off >= I.n) {
+    key[p] = -1;
+    return;
+  }
+  T x[3], fx[3];
+  int base[3];
+  ld3(dyn, n_slots, p, RX, x);
+  stencil_base(x, I.inv_dx, base, fx);
+  const int bx = base[0] + 2, by = base[1] + 2, bz = base[2] + 2;
+  const int BX = I.nx + 2, BY = I.ny + 2, BZ = I.nz + 2;
+  int64_t k;
+  if (bx >= 0 && bx < BX && by >= 0 && by < BY && bz >= 0 && bz < BZ)
+    k = I.c_off + ((int64_t)bx * BY + by) * BZ + bz;
+  else
+    k = I.c_off + (int64_t)BX * BY * BZ;  // dead bin: no in-grid node
+  key[p] = (int32_t)k;
+  slot[p] = atomicAdd(&count[k], 1);
+}
+
+// Exclusive scan of count[n] -> start[n + 1], in three passes.
+__global__ void k_scan_blocks(const int32_t* count, int32_t* start, int32_t* bsum, int64_t n) {
+  __shared__ int32_t sh[kScanBlock];
+  const int64_t i0 = ((int64_t)blockIdx.x * kScanBlock + threadIdx.x) * kScanPer;
+  int32_t v[kScanPer];
+  int32_t tot = 0;
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = (i0 + k < n) ? count[i0 + k] : 0;
+    tot += v[k];
+  }
+  sh[threadIdx.x] = tot;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    const int32_t add = (threadIdx.x >= (unsigned)off) ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  int32_t run = sh[threadIdx.x] - tot;  // exclusive prefix of this thread
+  for (int k = 0; k < kScanPer; ++k) {
+    if (i0 + k < n) start[i0 + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == kScanBlock - 1) bsum[blockIdx.x] = sh[threadIdx.x];
+}
+
+__global__ void k_scan_sums(int32_t* bsum, int64_t nb, int32_t* start, int64_t n) {
+  __shared__ int32_t sh[1024];
+  int32_t carry = 0;
+  for (int64_t c0 = 0; c0 < nb; c0 += 1024) {
+    const int64_t i = c0 + threadIdx.x;
+    const int32_t v = (i < nb) ? bsum[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int32_t add = (threadIdx.x >= (unsigned)off) ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (i < nb) bsum[i] = carry + sh[threadIdx.x] - v;
+    const int32_t chunk_tot = sh[1023];
+    __syncthreads();
+    carry += chunk_tot;
+  }
+  if (threadIdx.x == 0) start[n] = carry;
+}
+
+__global__ void k_scan_add(int32_t* start, const int32_t* bsum, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) start[i] += bsum[i / (kScanBlock * kScanPer)];
+}
+
+__global__ void k_place(const int32_t* key, const int32_t* slot, const int32_t* start,
+                        int32_t* sorted, int64_t n_slots) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_slots) return;
+  const int32_t k = key[p];
+  if (k < 0) return;
+  sorted[start[k] + slot[p]] = (int32_t)p;
+}
+
+// Insertion sort of each bin by slot (ascending), then rank[sorted[j]] = j.
+__global__ void k_bin_sort(const int32_t* start, int32_t* sorted, int32_t* rank, int64_t n_bins) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_bins) return;
+  const int32_t s0 = start[b], s1 = start[b + 1];
+  for (int32_t j = s0 + 1; j < s1; ++j) {
+    const int32_t v = sorted[j];
+    int32_t i = j - 1;
+    while (i >= s0 && sorted[i] > v) {
+      sorted[i + 1] = sorted[i];
+      --i;
+    }
+    sorted[i + 1] = v;
+  }
+  for (int32_t j = s0; j < s1; ++j) rank[sorted[j]] = j;
+}
+
+// ---------------------------------------------------------------------------
+// forward prep: stress, grown volume, APIC payload (transfer.py:43-45,53)
+//   m_i += w pmass;  mom_i += w (P + A dpos),  P = pmass v,  A = pmass C + dt c tau,
+//   c = -4 inv_dx^2 vol0 det Fg,  dpos = (o - fx) dx  ->  A dpos = A' (o - fx),
+//   A' = A dx;  stored: fx, Q = P - A' fx, A'.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_prep_fwd(const mpm_instance* inst, const int32_t* slot_inst, int64_t n_slots,
+                           const T* dyn, const T* Fgp, const T* sched, const int32_t* rank,
+                           T* payload) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_slots) return;
+  const int ii = slot_inst[p / kChunk];
+  const Inst<T> I = load_inst<T>(inst, ii);
+  if (p - I.p_off >= I.n) return;
+  T x[3], v[3], fx[3];
+  int base[3];
+  ld3(dyn, n_slots, p, RX, x);
+  ld3(dyn, n_slots, p, RV, v);
+  const M3<T> C = ld9(dyn, n_slots, p, RC);
+  const M3<T> F = ld9(dyn, n_slots, p, RF);
+  const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), sched ? sched[ii] : T(1));
+  stencil_base(x, I.inv_dx, base, fx);
+  const M3<T> tau = kirchhoff(I.mp, F, Fg, C);
+  const T c = -I.k4 * I.vol0 * det(Fg);
+  M3<T> Ap;
+  for (int k = 0; k < 9; ++k) Ap.a[k] = (I.pmass * C.a[k] + I.dt * (c * tau.a[k])) * I.dx;
+  const int64_t j = rank[p];
+  for (int a = 0; a < 3; ++a) {
+    const T q = I.pmass * v[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
+    payload[(int64_t)a * n_slots + j] = fx[a];
+    payload[(int64_t)(3 + a) * n_slots + j] = q;
+  }
+  for (int k = 0; k < 9; ++k) payload[(int64_t)(6 + k) * n_slots + j] = Ap.a[k];
+}
+
+// ---------------------------------------------------------------------------
+// P2G as a cell-binned segmented reduction into an LDS node tile.
+// Tile = TX*TY*TZ bins, one thread each; node tile (TX+2)(TY+2)(TZ+2).
+// partial: [n_tiles][4][TN] -- row 0 sum of weights (MASS), rows 1-3 sum of
+// w (Q + A' o).
+// ---------------------------------------------------------------------------
+template <typename T, int TX, int TY, int TZ, bool MASS>
+__global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* inst,
+                                                        const int32_t* tile_inst,
+                                                        const int32_t* bin_start,
+                                                        const T* payload, int64_t n_slots,
+                                                        T* partial) {
+  constexpr int NY = TY + 2, NZ = TZ + 2, TN = (TX + 2) * NY * NZ;
+  constexpr int NT = TX * TY * TZ;
+  __shared__ T lds[4 * TN];
+  const int64_t tile = blockIdx.x;
+  const int ii = tile_inst[tile];
+  const mpm_instance& I = inst[ii];
+  const int64_t lt = tile - I.t_off;
+  const int itz = (int)(lt % I.tz);
+  const int ity = (int)((lt / I.tz) % I.ty);
+  const int itx = (int)(lt / ((int64_t)I.tz * I.ty));
+  const int tid = threadIdx.x;
+  const int lz = tid % TZ, ly = (tid / TZ) % TY, lx = tid / (TY * TZ);
+  const int BX = I.nx + 2, BY = I.ny + 2, BZ = I.nz + 2;
+  const int bx = itx * TX + lx, by = ity * TY + ly, bz = itz * TZ + lz;
+
+  for (int i = tid; i < 4 * TN; i += NT) lds[i] = T(0);
+
+  T aw[27], am[27][3];
+#pragma unroll
+  for (int o = 0; o < 27; ++o) {
+    aw[o] = T(0);
+    am[o][0] = am[o][1] = am[o][2] = T(0);
+  }
+  if (bx < BX && by < BY && bz < BZ) {
+    const int64_t key = I.c_off + ((int64_t)bx * BY + by) * BZ + bz;
+    const int32_t s0 = bin_start[key], s1 = bin_start[key + 1];
+    for (int32_t j = s0; j < s1; ++j) {
+      T f[kPayload];
+#pragma unroll
+      for (int c = 0; c < kPayload; ++c) f[c] = payload[(int64_t)c * n_slots + j];
+      T w0[3], w1[3], w2[3];
+      bspline_w(f[0], w0);
+      bspline_w(f[1], w1);
+      bspline_w(f[2], w2);
+      // A' row-major at f[6..14]: A'(r, c) = f[6 + 3 r + c]
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+          const T wij = w0[i] * w1[jj];
+          const T u0 = f[3] + T(i) * f[6] + T(jj) * f[7];
+          const T u1 = f[4] + T(i) * f[9] + T(jj) * f[10];
+          const T u2 = f[5] + T(i) * f[12] + T(jj) * f[13];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const int o = (i * 3 + jj) * 3 + k;
+            const T wt = wij * w2[k];
+            if (MASS) aw[o] += wt;
+            am[o][0] += wt * (u0 + T(k) * f[8]);
+            am[o][1] += wt * (u1 + T(k) * f[11]);
+            am[o][2] += wt * (u2 + T(k) * f[14]);
+          }
+        }
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < 27; ++o) {
+    __syncthreads();
+    const int nl = ((lx + o / 9) * NY + (ly + (o / 3) % 3)) * NZ + (lz + o % 3);
+    if (MASS) lds[nl] += aw[o];
+    lds[TN + nl] += am[o][0];
+    lds[2 * TN + nl] += am[o][1];
+    lds[3 * TN + nl] += am[o][2];
+  }
+  __syncthreads();
+  T* out = partial + tile * (int64_t)(4 * TN);
+  for (int i = tid; i < 4 * TN; i += NT) out[i] = lds[i];
+}
+
+// Sum of the tile partials covering node (i, j, k), tiles in ascending order.
+template <typename T, int TX, int TY, int TZ>
+__device__ __forceinline__ void gather_tiles(const Inst<T>& I, const T* partial, int i, int j, int k,
+                                             T acc[4]) {
+  constexpr int NY = TY + 2, NZ = TZ + 2, TN = (TX + 2) * NY * NZ;
+  acc[0] = acc[1] = acc[2] = acc[3] = T(0);
+  const int xhi = min((i + 2) / TX, I.tx - 1), yhi = min((j + 2) / TY, I.ty - 1),
+            zhi = min((k + 2) / TZ, I.tz - 1);
+  for (int ix = max(0, xhi - 2); ix <= xhi; ++ix) {
+    const int li = i - (ix * TX - 2);
+    if (li < 0 || li >= TX + 2) continue;
+    for (int iy = max(0, yhi - 2); iy <= yhi; ++iy) {
+      const int lj = j - (iy * TY - 2);
+      if (lj < 0 || lj >= TY + 2) continue;
+      for (int iz = max(0, zhi - 2); iz <= zhi; ++iz) {
+        const int lk = k - (iz * TZ - 2);
+        if (lk < 0 || lk >= TZ + 2) continue;
+        const int64_t tile = I.t_off + ((int64_t)ix * I.ty + iy) * I.tz + iz;
+        const T* src = partial + tile * (int64_t)(4 * TN) + (li * NY + lj) * NZ + lk;
+        acc[0] += src[0];
+        acc[1] += src[TN];
+        acc[2] += src[2 * TN];
+        acc[3] += src[3 * TN];
+      }
+    }
+  }
+}
+
+// Grid update (transfer.py:56-59): v = (1 - damping) mom / m where m > 1e-12.
+template <typename T, int TX, int TY, int TZ>
+__global__ void k_grid_fwd(const mpm_instance* inst, const int32_t* node_inst, int64_t n_nodes,
+                           const T* partial, T* grid) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_nodes) return;
+  const Inst<T> I = load_inst<T>(inst, node_inst[g / kChunk]);
+  const int64_t local = g - I.g_off;
+  if (local >= (int64_t)I.nx * I.ny * I.nz) return;
+  const int k = (int)(local % I.nz), j = (int)((local / I.nz) % I.ny),
+            i = (int)(local / ((int64_t)I.nz * I.ny));
+  T acc[4];
+  gather_tiles<T, TX, TY, TZ>(I, partial, i, j, k, acc);
+  const T m = I.pmass * acc[0];
+  T v[3] = {T(0), T(0), T(0)};
+  if (m > T(kMassEps))
+    for (int a = 0; a < 3; ++a) v[a] = I.s * acc[1 + a] / m;
+  grid[g] = m;
+  for (int a = 0; a < 3; ++a) grid[(int64_t)(1 + a) * n_nodes + g] = v[a];
+}
+
+// ---------------------------------------------------------------------------
+// G2P (transfer.py:61-77)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int32_t* slot_inst, int64_t n_slots,
+                      int64_t n_nodes, const T* dyn_in, const T* grid, T* dyn_out, int advect) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_slots) return;
+  const Inst<T> I = load_inst<T>(inst, slot_inst[p / kChunk]);
+  if (p - I.p_off >= I.n) return;
+  T x[3], fx[3];
+  int base[3];
+  ld3(dyn_in, n_slots, p, RX, x);
+  const M3<T> F = ld9(dyn_in, n_slots, p, RF);
+  stencil_base(x, I.inv_dx, base, fx);
+  T w[3][3];
+  for (int a = 0; a < 3; ++a) bspline_w(fx[a], w[a]);
+  T nv[3] = {T(0), T(0), T(0)};
+  M3<T> nC = zero3<T>();
+  for (int i = 0; i < 3; ++i) {
+    const int ni = base[0] + i;
+    if (ni < 0 || ni >= I.nx) continue;
+    for (int j = 0; j < 3; ++j) {
+      const int nj = base[1] + j;
+      if (nj < 0 || nj >= I.ny) continue;
+      for (int k = 0; k < 3; ++k) {
+        const int nk = base[2] + k;
+        if (nk < 0 || nk >= I.nz) continue;
+        const int64_t gi = I.g_off + ((int64_t)ni * I.ny + nj) * I.nz + nk;
+        const T wt = w[0][i] * w[1][j] * w[2][k];
+        const T d[3] = {(T(i) - fx[0]) * I.dx, (T(j) - fx[1]) * I.dx, (T(k) - fx[2]) * I.dx};
+        T vi[3];
+        for (int a = 0; a < 3; ++a) vi[a] = grid[(int64_t)(1 + a) * n_nodes + gi];
+        const T sw = I.k4 * wt;
+        for (int a = 0; a < 3; ++a) {
+          nv[a] += wt * vi[a];
+          for (int b = 0; b < 3; ++b) nC(a, b) += sw * (vi[a] * d[b]);
+        }
+      }
+    }
+  }
+  M3<T> A = scaled(nC, I.dt);
+  A.a[0] += T(1); A.a[4] += T(1); A.a[8] += T(1);
+  const M3<T> Fn = mul(A, F);
+  T xn[3];
+  for (int a = 0; a < 3; ++a) xn[a] = advect ? x[a] + I.dt * nv[a] : x[a];
+  st3(dyn_out, n_slots, p, RX, xn);
+  st3(dyn_out, n_slots, p, RV, nv);
+  st9(dyn_out, n_slots, p, RC, nC);
+  st9(dyn_out, n_slots, p, RF, Fn);
+}
+
+// ---------------------------------------------------------------------------
+// reverse step, part 1 (transfer.py:100-114): G2P-adjoint payload.
+//   gv_bar_i += w nv_b + 4 inv_dx^2 w C_b' dpos,  nv_b = vn_b + dt xn_b,
+//   C_b' = Cn_b + dt Fn_b F^T;  stored like the forward payload with
+//   P = nv_b, A = 4 inv_dx^2 C_b'.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_prep_adj(const mpm_instance* inst, const int32_t* slot_inst, int64_t n_slots,
+                           const T* dyn_in, const T* bar_out, const int32_t* rank, T* payload,
+                           int advect) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_slots) return;
+  const Inst<T> I = load_inst<T>(inst, slot_inst[p / kChunk]);
+  if (p - I.p_off >= I.n) return;
+  T x[3], fx[3], nvb[3];
+  int base[3];
+  ld3(dyn_in, n_slots, p, RX, x);
+  stencil_base(x, I.inv_dx, base, fx);
+  ld3(bar_out, n_slots, p, RV, nvb);
+  if (advect) {
+    T xb[3];
+    ld3(bar_out, n_slots, p, RX, xb);
+    for (int a = 0; a < 3; ++a) nvb[a] += I.dt * xb[a];
+  }
+  const M3<T> F = ld9(dyn_in, n_slots, p, RF);
+  const M3<T> Cnb = ld9(bar_out, n_slots, p, RC);
+  const M3<T> Fnb = ld9(bar_out, n_slots, p, RF);
+  const M3<T> FF = mul_nt(Fnb, F);
+  M3<T> Ap;
+  for (int k = 0; k < 9; ++k) Ap.a[k] = I.k4 * (Cnb.a[k] + I.dt * FF.a[k]) * I.dx;
+  const int64_t j = rank[p];
+  for (int a = 0; a < 3; ++a) {
+    const T q = nvb[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
+    payload[(int64_t)a * n_slots + j] = fx[a];
+    payload[(int64_t)(3 + a) * n_slots + j] = q;
+  }
+  for (int k = 0; k < 9; ++k) payload[(int64_t)(6 + k) * n_slots + j] = Ap.a[k];
+}
+
+// reverse step, part 2 (transfer.py:121-127): grid adjoint.
+//   gadj row 0 = gm_bar, rows 1-3 = gmom_bar.
+template <typename T, int TX, int TY, int TZ>
+__global__ void k_grid_adj(const mpm_instance* inst, const int32_t* node_inst, int64_t n_nodes,
+                           const T* partial, const T* grid, T* gadj, int advect) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_nodes) return;
+  const Inst<T> I = load_inst<T>(inst, node_inst[g / kChunk]);
+  const int64_t local = g - I.g_off;
+  if (local >= (int64_t)I.nx * I.ny * I.nz) return;
+  const int k = (int)(local % I.nz), j = (int)((local / I.nz) % I.ny),
+            i = (int)(local / ((int64_t)I.nz * I.ny));
+  T acc[4];
+  gather_tiles<T, TX, TY, TZ>(I, partial, i, j, k, acc);
+  const T m = grid[g];
+  T mb[3] = {T(0), T(0), T(0)};
+  T gmb = T(0);
+  if (m > T(kMassEps)) {
+    for (int a = 0; a < 3; ++a) mb[a] = I.s * acc[1 + a] / m;
+    if (advect) {
+      T dot = T(0);
+      for (int a = 0; a < 3; ++a) dot += acc[1 + a] * grid[(int64_t)(1 + a) * n_nodes + g];
+      gmb = -dot / m;
+    }
+  }
+  gadj[g] = gmb;
+  for (int a = 0; a < 3; ++a) gadj[(int64_t)(1 + a) * n_nodes + g] = mb[a];
+}
+
+// reverse step, part 3 (transfer.py:100-119 x_bar terms, :129-156):
+// P2G-adjoint gather, advection terms, constitutive VJP, Fg and theta paths.
+template <typename T>
+__global__ __launch_bounds__(256) void k_p2g_adj(const mpm_instance* inst, const int32_t* slot_inst,
+                                                 int64_t n_slots, int64_t n_nodes, const T* dyn_in,
+                                                 const T* Fgp, const T* sched, const T* dyn_out,
+                                                 const T* grid, const T* gadj, const T* bar_out,
+                                                 T* bar_in, T* Fg_bar, double* theta_acc,
+                                                 int advect) {
+  __shared__ double red[2][256];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t pc = (p < n_slots) ? p : n_slots - 1;
+  const int ii = slot_inst[pc / kChunk];
+  const Inst<T> I = load_inst<T>(inst, ii);
+  const bool live = (p < n_slots) && (p - I.p_off < I.n);
+  T th[2] = {T(0), T(0)};
+  if (live) {
+    T x[3], v[3], fx[3];
+    int base[3];
+    ld3(dyn_in, n_slots, p, RX, x);
+    ld3(dyn_in, n_slots, p, RV, v);
+    const M3<T> C = ld9(dyn_in, n_slots, p, RC);
+    const M3<T> F = ld9(dyn_in, n_slots, p, RF);
+    const T a_t = sched ? sched[ii] : T(1);
+    const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), a_t);
+    const M3<T> Cn = ld9(dyn_out, n_slots, p, RC);
+    stencil_base(x, I.inv_dx, base, fx);
+    T w[3][3], dw[3][3];
+    for (int a = 0; a < 3; ++a) {
+      bspline_w(fx[a], w[a]);
+      bspline_dw(fx[a], dw[a]);
+    }
+    const M3<T> tau = kirchhoff(I.mp, F, Fg, C);
+    const T detG = det(Fg);
+    const T c = -I.k4 * I.vol0 * detG;
+    const M3<T> sk = scaled(tau, c);
+    // A = pmass C + dt sk (momentum affine map of the forward P2G)
+    M3<T> A;
+    for (int k = 0; k < 9; ++k) A.a[k] = I.pmass * C.a[k] + I.dt * sk.a[k];
+
+    // cotangents of the step outputs
+    T xnb[3], nvb[3];
+    ld3(bar_out, n_slots, p, RX, xnb);
+    ld3(bar_out, n_slots, p, RV, nvb);
+    if (advect)
+      for (int a = 0; a < 3; ++a) nvb[a] += I.dt * xnb[a];
+    const M3<T> Cnb = ld9(bar_out, n_slots, p, RC);
+    const M3<T> Fnb = ld9(bar_out, n_slots, p, RF);
+    M3<T> Cb = mul_nt(Fnb, F);
+    for (int k = 0; k < 9; ++k) Cb.a[k] = Cnb.a[k] + I.dt * Cb.a[k];
+    M3<T> IC = scaled(Cn, I.dt);
+    IC.a[0] += T(1); IC.a[4] += T(1); IC.a[8] += T(1);
+    M3<T> Fb = mul_tn(IC, Fnb);  // (I + dt Cn)^T Fn_b
+
+    T xb[3] = {T(0), T(0), T(0)};
+    if (advect)
+      for (int a = 0; a < 3; ++a) xb[a] = xnb[a];
+    T vb[3] = {T(0), T(0), T(0)};
+    M3<T> MD = zero3<T>();
+    T Sv[3] = {T(0), T(0), T(0)};  // sum w v_i
+    T Sm[3] = {T(0), T(0), T(0)};  // sum w mb_i
+
+    for (int i = 0; i < 3; ++i) {
+      const int ni = base[0] + i;
+      if (ni < 0 || ni >= I.nx) continue;
+      for (int j = 0; j < 3; ++j) {
+        const int nj = base[1] + j;
+        if (nj < 0 || nj >= I.ny) continue;
+        for (int k = 0; k < 3; ++k) {
+          const int nk = base[2] + k;
+          if (nk < 0 || nk >= I.nz) continue;
+          const int64_t gi = I.g_off + ((int64_t)ni * I.ny + nj) * I.nz + nk;
+          const T wt = w[0][i] * w[1][j] * w[2][k];
+          const T d[3] = {(T(i) - fx[0]) * I.dx, (T(j) - fx[1]) * I.dx, (T(k) - fx[2]) * I.dx};
+          T mb[3];
+          for (int a = 0; a < 3; ++a) mb[a] = gadj[(int64_t)(1 + a) * n_nodes + gi];
+          for (int a = 0; a < 3; ++a) {
+            vb[a] += wt * mb[a];
+            for (int b = 0; b < 3; ++b) MD(a, b) += wt * (mb[a] * d[b]);
+          }
+          if (advect) {
+            T vi[3];
+            for (int a = 0; a < 3; ++a) vi[a] = grid[(int64_t)(1 + a) * n_nodes + gi];
+            const T gmb = gadj[gi];
+            const T gw[3] = {dw[0][i] * w[1][j] * w[2][k] * I.inv_dx,
+                             w[0][i] * dw[1][j] * w[2][k] * I.inv_dx,
+                             w[0][i] * w[1][j] * dw[2][k] * I.inv_dx};
+            // G2P adjoint:  (nv_b . v_i) + 4 inv_dx^2 <C_b', v_i dpos^T>
+            // P2G adjoint:  mb . (pmass v + A dpos) + gm_b pmass
+            T alpha = gmb * I.pmass;
+            for (int a = 0; a < 3; ++a) {
+              const T Cd = Cb(a, 0) * d[0] + Cb(a, 1) * d[1] + Cb(a, 2) * d[2];
+              const T Ad = A(a, 0) * d[0] + A(a, 1) * d[1] + A(a, 2) * d[2];
+              alpha += nvb[a] * vi[a] + I.k4 * vi[a] * Cd + mb[a] * (I.pmass * v[a] + Ad);
+            }
+            for (int a = 0; a < 3; ++a) {
+              xb[a] += alpha * gw[a];
+              Sv[a] += wt * vi[a];
+              Sm[a] += wt * mb[a];
+            }
+          }
+        }
+      }
+    }
+    if (advect) {
+      // - 4 inv_dx^2 C_b'^T (sum w v_i) - A^T (sum w mb_i)
+      for (int a = 0; a < 3; ++a)
+        xb[a] -= I.k4 * (Cb(0, a) * Sv[0] + Cb(1, a) * Sv[1] + Cb(2, a) * Sv[2]) +
+                 (A(0, a) * Sm[0] + A(1, a) * Sm[1] + A(2, a) * Sm[2]);
+    }
+    for (int a = 0; a < 3; ++a) vb[a] *= I.pmass;
+    M3<T> Cbo = scaled(MD, I.pmass);
+    const M3<T> skb = scaled(MD, I.dt);
+    const M3<T> taub = scaled(skb, c);
+    M3<T> Fpb, Fgb, Cbar;
+    kirchhoff_vjp(I.mp, F, Fg, C, taub, Fpb, Fgb, Cbar, th);
+    for (int k = 0; k < 9; ++k) {
+      Fb.a[k] += Fpb.a[k];
+      Cbo.a[k] += Cbar.a[k];
+    }
+    // grown-volume path: <sk_b, tau> (-k4 vol0) det Fg Fg^{-T}
+    const M3<T> GiT = inv_t(Fg);
+    const T vol = frob(skb, tau) * (-I.k4 * I.vol0) * detG;
+    M3<T> Gb = ld9(Fg_bar, n_slots, p, 0);
+    for (int k = 0; k < 9; ++k) Gb.a[k] += a_t * (Fgb.a[k] + vol * GiT.a[k]);
+    st3(bar_in, n_slots, p, RX, xb);
+    st3(bar_in, n_slots, p, RV, vb);
+    st9(bar_in, n_slots, p, RC, Cbo);
+    st9(bar_in, n_slots, p, RF, Fb);
+    st9(Fg_bar, n_slots, p, 0, Gb);
+  }
+  if (theta_acc != nullptr) {  // fixed-order block reduction, one slot per 256 particles
+    red[0][threadIdx.x] = (double)th[0];
+    red[1][threadIdx.x] = (double)th[1];
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < (unsigned)off) {
+        red[0][threadIdx.x] += red[0][threadIdx.x + off];
+        red[1][threadIdx.x] += red[1][threadIdx.x + off];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0 && p < n_slots) {
+      theta_acc[2 * blockIdx.x] += red[0][0];
+      theta_acc[2 * blockIdx.x + 1] += red[1][0];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// constitutive seam kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_stress(ModelParams<T> mp, int64_t n, int64_t stride, const T* F, const T* Fg,
+                         const T* C, T* tau) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const M3<T> Cm = C ? ld9(C, stride, p, 0) : zero3<T>();
+  st9(tau, stride, p, 0, kirchhoff(mp, ld9(F, stride, p, 0), ld9(Fg, stride, p, 0), Cm));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_stress_vjp(ModelParams<T> mp, int64_t n, int64_t stride, const T* F,
+                             const T* Fg, const T* C, const T* tb, T* Fb, T* Fgb, T* Cb, T* thb) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const M3<T> Cm = C ? ld9(C, stride, p, 0) : zero3<T>();
+  M3<T> fb, gb, cb;
+  T th[2];
+  kirchhoff_vjp(mp, ld9(F, stride, p, 0), ld9(Fg, stride, p, 0), Cm, ld9(tb, stride, p, 0), fb, gb,
+                cb, th);
+  st9(Fb, stride, p, 0, fb);
+  st9(Fgb, stride, p, 0, gb);
+  if (Cb) st9(Cb, stride, p, 0, cb);
+  if (thb) {
+    thb[p] = th[0];
+    thb[stride + p] = th[1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct TileShape {
+  int x, y, z;
+};
+constexpr TileShape kTiles[] = {{4, 8, 8}, {4, 4, 16}, {8, 8, 4}, {4, 4, 8}};
+constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+
+int tile_index(const mpm_batch* b) {
+  for (int i = 0; i < kNumTiles; ++i)
+    if (kTiles[i].x == b->tile[0] && kTiles[i].y == b->tile[1] && kTiles[i].z == b->tile[2]) return i;
+  return -1;
+}
+
+int64_t tile_nodes(const mpm_batch* b) {
+  return (int64_t)(b->tile[0] + 2) * (b->tile[1] + 2) * (b->tile[2] + 2);
+}
+
+inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+struct Workspace {
+  int32_t *key, *slot, *count, *bsum;
+  void *payload, *partial, *gadj;
+};
+
+int64_t scan_blocks(int64_t n_bins) {
+  return (n_bins + kScanBlock * kScanPer - 1) / (kScanBlock * kScanPer);
+}
+
+int64_t carve(const mpm_batch* b, size_t tsz, char* base, Workspace* w) {
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) {
+    char* p = base ? base + off : nullptr;
+    off = align_up(off + bytes, 256);
+    return p;
+  };
+  Workspace ws;
+  ws.key = (int32_t*)take(4 * b->n_slots);
+  ws.slot = (int32_t*)take(4 * b->n_slots);
+  ws.count = (int32_t*)take(4 * b->n_bins);
+  ws.bsum = (int32_t*)take(4 * (scan_blocks(b->n_bins) + 1));
+  ws.payload = take((int64_t)tsz * kPayload * b->n_slots);
+  ws.partial = take((int64_t)tsz * 4 * tile_nodes(b) * b->n_tiles);
+  ws.gadj = take((int64_t)tsz * 4 * b->n_nodes);
+  if (w) *w = ws;
+  return off;
+}
+
+int validate(const mpm_batch* b) {
+  if (!b) return set_err(MPM_EINVAL, "batch is NULL");
+  if (b->n_inst <= 0 || !b->inst || !b->slot_inst || !b->node_inst || !b->tile_inst)
+    return set_err(MPM_EINVAL, "batch tables missing");
+  if (b->n_slots <= 0 || b->n_slots % kChunk || b->n_nodes <= 0 || b->n_nodes % kChunk)
+    return set_err(MPM_EINVAL, "n_slots / n_nodes must be positive multiples of %d", kChunk);
+  if (b->n_slots >= (int64_t)1 << 31 || b->n_bins >= (int64_t)1 << 31)
+    return set_err(MPM_EINVAL, "n_slots and n_bins must be below 2^31");
+  if (b->n_bins <= 0 || b->n_tiles <= 0) return set_err(MPM_EINVAL, "empty bin or tile table");
+  if (b->block != 64 && b->block != 128 && b->block != 256)
+    return set_err(MPM_EINVAL, "block must be 64, 128 or 256 (got %d)", b->block);
+  if (tile_index(b) < 0)
+    return set_err(MPM_ENOTSUP, "tile shape %dx%dx%d not built", b->tile[0], b->tile[1], b->tile[2]);
+  return MPM_OK;
+}
+
+inline dim3 grid1(int64_t n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+template <typename T>
+int sort_impl(const mpm_batch* b, const T* dyn, int32_t* sortbuf, const Workspace& w,
+              hipStream_t st) {
+  int32_t* rank = sortbuf;
+  int32_t* sorted = sortbuf + b->n_slots;
+  int32_t* start = sortbuf + 2 * b->n_slots;
+  const int bs = b->block;
+  k_zero_i32<<<grid1(b->n_bins, 256), 256, 0, st>>>(w.count, b->n_bins);
+  k_sort_key<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn,
+                                                       w.key, w.slot, w.count);
+  const int64_t nb = scan_blocks(b->n_bins);
+  k_scan_blocks<<<dim3((unsigned)nb), kScanBlock, 0, st>>>(w.count, start, w.bsum, b->n_bins);
+  k_scan_sums<<<1, 1024, 0, st>>>(w.bsum, nb, start, b->n_bins);
+  k_scan_add<<<grid1(b->n_bins, 256), 256, 0, st>>>(start, w.bsum, b->n_bins);
+  k_place<<<grid1(b->n_slots, bs), bs, 0, st>>>(w.key, w.slot, start, sorted, b->n_slots);
+  k_bin_sort<<<grid1(b->n_bins, 64), 64, 0, st>>>(start, sorted, rank, b->n_bins);
+  return check_launch("sort_by_cell");
+}
+
+template <typename T, int TX, int TY, int TZ>
+void launch_tile_fwd(const mpm_batch* b, const int32_t* start, const Workspace& w, T* grid,
+                     hipStream_t st) {
+  k_p2g_tile<T, TX, TY, TZ, true><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
+      b->inst, b->tile_inst, start, (const T*)w.payload, b->n_slots, (T*)w.partial);
+  k_grid_fwd<T, TX, TY, TZ><<<grid1(b->n_nodes, 256), 256, 0, st>>>(
+      b->inst, b->node_inst, b->n_nodes, (const T*)w.partial, grid);
+}
+
+template <typename T, int TX, int TY, int TZ>
+void launch_tile_adj(const mpm_batch* b, const int32_t* start, const Workspace& w, const T* grid,
+                     int advect, hipStream_t st) {
+  k_p2g_tile<T, TX, TY, TZ, false><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
+      b->inst, b->tile_inst, start, (const T*)w.payload, b->n_slots, (T*)w.partial);
+  k_grid_adj<T, TX, TY, TZ><<<grid1(b->n_nodes, 256), 256, 0, st>>>(
+      b->inst, b->node_inst, b->n_nodes, (const T*)w.partial, grid, (T*)w.gadj, advect);
+}
+
+template <typename T>
+void dispatch_fwd(int ti, const mpm_batch* b, const int32_t* start, const Workspace& w, T* grid,
+                  hipStream_t st) {
+  switch (ti) {
+    case 0: launch_tile_fwd<T, 4, 8, 8>(b, start, w, grid, st); break;
+    case 1: launch_tile_fwd<T, 4, 4, 16>(b, start, w, grid, st); break;
+    case 2: launch_tile_fwd<T, 8, 8, 4>(b, start, w, grid, st); break;
+    default: launch_tile_fwd<T, 4, 4, 8>(b, start, w, grid, st); break;
+  }
+}
+
+template <typename T>
+void dispatch_adj(int ti, const mpm_batch* b, const int32_t* start, const Workspace& w,
+                  const T* grid, int advect, hipStream_t st) {
+  switch (ti) {
+    case 0: launch_tile_adj<T, 4, 8, 8>(b, start, w, grid, advect, st); break;
+    case 1: launch_tile_adj<T, 4, 4, 16>(b, start, w, grid, advect, st); break;
+    case 2: launch_tile_adj<T, 8, 8, 4>(b, start, w, grid, advect, st); break;
+    default: launch_tile_adj<T, 4, 4, 8>(b, start, w, grid, advect, st); break;
+  }
+}
+
+template <typename T>
+int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sched, T* dyn_out,
+                  T* grid, int32_t* sortbuf, int advect, void* ws, void* stream) {
+  int rc = validate(b);
+  if (rc) return rc;
+  if (!dyn_in || !Fg || !dyn_out || !grid || !sortbuf || !ws)
+    return set_err(MPM_EINVAL, "step_fwd: NULL buffer");
+  if (dyn_in == dyn_out) return set_err(MPM_EINVAL, "step_fwd: dyn_out must not alias dyn_in");
+  hipStream_t st = (hipStream_t)stream;
+  Workspace w;
+  carve(b, sizeof(T), (char*)ws, &w);
+  rc = sort_impl<T>(b, dyn_in, sortbuf, w, st);
+  if (rc) return rc;
+  const int32_t* rank = sortbuf;
+  const int32_t* start = sortbuf + 2 * b->n_slots;
+  const int bs = b->block;
+  k_prep_fwd<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in,
+                                                      Fg, sched, rank, (T*)w.payload);
+  dispatch_fwd<T>(tile_index(b), b, start, w, grid, st);
+  k_g2p<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, b->n_nodes,
+                                                 dyn_in, grid, dyn_out, advect);
+  return check_launch("step_fwd");
+}
+
+template <typename T>
+int step_bwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sched,
+                  const T* dyn_out, const T* grid, const int32_t* sortbuf, const T* bar_out,
+                  T* bar_in, T* Fg_bar, double* theta_acc, int advect, void* ws, void* stream) {
+  int rc = validate(b);
+  if (rc) return rc;
+  if (!dyn_in || !Fg || !dyn_out || !grid || !sortbuf || !bar_out || !bar_in || !Fg_bar || !ws)
+    return set_err(MPM_EINVAL, "step_bwd: NULL buffer");
+  if (bar_in == bar_out) return set_err(MPM_EINVAL, "step_bwd: bar_in must not alias bar_out");
+  hipStream_t st = (hipStream_t)stream;
+  Workspace w;
+  carve(b, sizeof(T), (char*)ws, &w);
+  const int32_t* rank = sortbuf;
+  const int32_t* start = sortbuf + 2 * b->n_slots;
+  const int bs = b->block;
+  k_prep_adj<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in,
+                                                      bar_out, rank, (T*)w.payload, advect);
+  dispatch_adj<T>(tile_index(b), b, start, w, grid, advect, st);
+  k_p2g_adj<T><<<grid1(b->n_slots, 256), 256, 0, st>>>(
+      b->inst, b->slot_inst, b->n_slots, b->n_nodes, dyn_in, Fg, sched, dyn_out, grid,
+      (const T*)w.gadj, bar_out, bar_in, Fg_bar, theta_acc, advect);
+  return check_launch("step_bwd");
+}
+
+template <typename T>
+int stress_impl(int32_t model, const double* prm6, int64_t n, int64_t stride, const T* F,
+                const T* Fg, const T* C, T* tau, void* stream) {
+  if (model < 0 || model > 2) return set_err(MPM_EINVAL, "unknown model %d", model);
+  if (!prm6 || !F || !Fg || !tau || n < 0 || stride < n)
+    return set_err(MPM_EINVAL, "stress: bad arguments");
+  if (model == MPM_HERSCHEL_BULKLEY && !C) return set_err(MPM_EINVAL, "stress: HB needs C");
+  if (n == 0) return MPM_OK;
+  ModelParams<T> mp;
+  mp.kind = model;
+  for (int k = 0; k < 6; ++k) mp.p[k] = (T)prm6[k];
+  k_stress<T><<<grid1(n, 256), 256, 0, (hipStream_t)stream>>>(mp, n, stride, F, Fg, C, tau);
+  return check_launch("stress");
+}
+
+template <typename T>
+int stress_vjp_impl(int32_t model, const double* prm6, int64_t n, int64_t stride, const T* F,
+                    const T* Fg, const T* C, const T* tb, T* Fb, T* Fgb, T* Cb, T* thb,
+                    void* stream) {
+  if (model < 0 || model > 2) return set_err(MPM_EINVAL, "unknown model %d", model);
+  if (!prm6 || !F || !Fg || !tb || !Fb || !Fgb || n < 0 || stride < n)
+    return set_err(MPM_EINVAL, "stress_vjp: bad arguments");
+  if (model == MPM_HERSCHEL_BULKLEY && !C) return set_err(MPM_EINVAL, "stress_vjp: HB needs C");
+  if (n == 0) return MPM_OK;
+  ModelParams<T> mp;
+  mp.kind = model;
+  for (int k = 0; k < 6; ++k) mp.p[k] = (T)prm6[k];
+  k_stress_vjp<T><<<grid1(n, 256), 256, 0, (hipStream_t)stream>>>(mp, n, stride, F, Fg, C, tb, Fb,
+                                                                   Fgb, Cb, thb);
+  return check_launch("stress_vjp");
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mpm_abi_version(void) { return MPM_ABI_VERSION; }
+const char* mpm_last_error(void) { return g_err; }
+
+int64_t mpm_workspace_bytes(const mpm_batch* b, int32_t dtype) {
+  const int rc = validate(b);
+  if (rc) return rc;
+  if (dtype != 0 && dtype != 1) return set_err(MPM_EINVAL, "dtype must be 0 (f32) or 1 (f64)");
+  return carve(b, dtype ? sizeof(double) : sizeof(float), nullptr, nullptr);
+}
+
+int32_t mpm_tile_shapes(int32_t* out, int32_t max_shapes) {
+  const int n = kNumTiles < max_shapes ? kNumTiles : max_shapes;
+  for (int i = 0; i < n && out; ++i) {
+    out[3 * i] = kTiles[i].x;
+    out[3 * i + 1] = kTiles[i].y;
+    out[3 * i + 2] = kTiles[i].z;
+  }
+  return kNumTiles;
+}
+
+int32_t mpm_stress_f32(int32_t model, const double* prm6, int64_t n, int64_t stride, const float* F,
+                       const float* Fg, const float* C, float* tau, void* stream) {
+  return stress_impl<float>(model, prm6, n, stride, F, Fg, C, tau, stream);
+}
+int32_t mpm_stress_f64(int32_t model, const double* prm6, int64_t n, int64_t stride,
+                       const double* F, const double* Fg, const double* C, double* tau,
+                       void* stream) {
+  return stress_impl<double>(model, prm6, n, stride, F, Fg, C, tau, stream);
+}
+int32_t mpm_stress_vjp_f32(int32_t model, const double* prm6, int64_t n, int64_t stride,
+                           const float* F, const float* Fg, const float* C, const float* tau_bar,
+                           float* F_bar, float* Fg_bar, float* C_bar, float* theta_bar,
+                           void* stream) {
+  return stress_vjp_impl<float>(model, prm6, n, stride, F, Fg, C, tau_bar, F_bar, Fg_bar, C_bar,
+                                theta_bar, stream);
+}
+int32_t mpm_stress_vjp_f64(int32_t model, const double* prm6, int64_t n, int64_t stride,
+                           const double* F, const double* Fg, const double* C,
+                           const double* tau_bar, double* F_bar, double* Fg_bar, double* C_bar,
+                           double* theta_bar, void* stream) {
+  return stress_vjp_impl<double>(model, prm6, n, stride, F, Fg, C, tau_bar, F_bar, Fg_bar, C_bar,
+                                 theta_bar, stream);
+}
+
+int32_t mpm_sort_by_cell_f32(const mpm_batch* b, const float* dyn, int32_t* sortbuf, void* ws,
+                             void* stream) {
+  int rc = validate(b);
+  if (rc) return rc;
+  if (!dyn || !sortbuf || !ws) return set_err(MPM_EINVAL, "sort_by_cell: NULL buffer");
+  Workspace w;
+  carve(b, sizeof(float), (char*)ws, &w);
+  return sort_impl<float>(b, dyn, sortbuf, w, (hipStream_t)stream);
+}
+int32_t mpm_sort_by_cell_f64(const mpm_batch* b, const double* dyn, int32_t* sortbuf, void* ws,
+                             void* stream) {
+  int rc = validate(b);
+  if (rc) return rc;
+  if (!dyn || !sortbuf || !ws) return set_err(MPM_EINVAL, "sort_by_cell: NULL buffer");
+  Workspace w;
+  carve(b, sizeof(double), (char*)ws, &w);
+  return sort_impl<double>(b, dyn, sortbuf, w, (hipStream_t)stream);
+}
+
+int32_t mpm_step_fwd_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
+                         const float* sched, float* dyn_out, float* grid, int32_t* sortbuf,
+                         int32_t advect, void* ws, void* stream) {
+  return step_fwd_impl<float>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, advect, ws, stream);
+}
+int32_t mpm_step_fwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
+                         const double* sched, double* dyn_out, double* grid, int32_t* sortbuf,
+                         int32_t advect, void* ws, void* stream) {
+  return step_fwd_impl<double>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, advect, ws, stream);
+}
+int32_t mpm_step_bwd_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
+                         const float* sched, const float* dyn_out, const float* grid,
+                         const int32_t* sortbuf, const float* bar_out, float* bar_in,
+                         float* Fg_bar, double* theta_acc, int32_t advect, void* ws,
+                         void* stream) {
+  return step_bwd_impl<float>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, bar_out, bar_in,
+                              Fg_bar, theta_acc, advect, ws, stream);
+}
+int32_t mpm_step_bwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
+                         const double* sched, const double* dyn_out, const double* grid,
+                         const int32_t* sortbuf, const double* bar_out, double* bar_in,
+                         double* Fg_bar, double* theta_acc, int32_t advect, void* ws,
+                         void* stream) {
+  return step_bwd_impl<double>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, bar_out, bar_in,
+                               Fg_bar, theta_acc, advect, ws, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,390 @@
+// Register-resident 3x3 arithmetic and the constitutive models of the MLS-MPM
+// growth step, templated on the working precision T (float or double).
+//
+// Every function is __host__ __device__: the HIP kernels use them, and the
+// host build (g++ on mpm_host.cpp) unit-tests the same code on the CPU.
+//
+// Matrices are row-major 3x3, element (r, c) at a[3 r + c] -- the component
+// order of the reference's (n, 3, 3) arrays (python/morphompm/state.py:12-16).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <limits>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MPM_HD __host__ __device__ __forceinline__
+#else
+#define MPM_HD inline
+#endif
+
+namespace mpm {
+
+enum Model : int32_t { NEO_HOOKEAN = 0, HENCKY = 1, HERSCHEL_BULKLEY = 2 };
+
+// Reference thresholds (python/morphompm/constitutive.py:17,
+// python/morphompm/transfer.py:57,123).
+constexpr double kDegen = 1.0e-9;
+constexpr double kMassEps = 1.0e-12;
+
+template <typename T>
+struct M3 {
+  T a[9];
+  MPM_HD T& operator()(int r, int c) { return a[3 * r + c]; }
+  MPM_HD const T& operator()(int r, int c) const { return a[3 * r + c]; }
+};
+
+template <typename T> MPM_HD M3<T> zero3() { M3<T> m; for (int i = 0; i < 9; ++i) m.a[i] = T(0); return m; }
+template <typename T> MPM_HD M3<T> eye3() { M3<T> m = zero3<T>(); m.a[0] = m.a[4] = m.a[8] = T(1); return m; }
+
+template <typename T> MPM_HD M3<T> mul(const M3<T>& A, const M3<T>& B) {
+  M3<T> C;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      C(r, c) = A(r, 0) * B(0, c) + A(r, 1) * B(1, c) + A(r, 2) * B(2, c);
+  return C;
+}
+// A^T B
+template <typename T> MPM_HD M3<T> mul_tn(const M3<T>& A, const M3<T>& B) {
+  M3<T> C;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      C(r, c) = A(0, r) * B(0, c) + A(1, r) * B(1, c) + A(2, r) * B(2, c);
+  return C;
+}
+// A B^T
+template <typename T> MPM_HD M3<T> mul_nt(const M3<T>& A, const M3<T>& B) {
+  M3<T> C;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      C(r, c) = A(r, 0) * B(c, 0) + A(r, 1) * B(c, 1) + A(r, 2) * B(c, 2);
+  return C;
+}
+template <typename T> MPM_HD M3<T> transpose(const M3<T>& A) {
+  M3<T> B;
+  for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) B(r, c) = A(c, r);
+  return B;
+}
+template <typename T> MPM_HD M3<T> scaled(const M3<T>& A, T s) {
+  M3<T> B; for (int i = 0; i < 9; ++i) B.a[i] = A.a[i] * s; return B;
+}
+template <typename T> MPM_HD M3<T> sym(const M3<T>& A) {
+  M3<T> B;
+  for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) B(r, c) = T(0.5) * (A(r, c) + A(c, r));
+  return B;
+}
+template <typename T> MPM_HD T trace(const M3<T>& A) { return A.a[0] + A.a[4] + A.a[8]; }
+template <typename T> MPM_HD T frob(const M3<T>& A, const M3<T>& B) {
+  T s = T(0); for (int i = 0; i < 9; ++i) s += A.a[i] * B.a[i]; return s;
+}
+template <typename T> MPM_HD T det(const M3<T>& A) {
+  return A.a[0] * (A.a[4] * A.a[8] - A.a[5] * A.a[7])
+       - A.a[1] * (A.a[3] * A.a[8] - A.a[5] * A.a[6])
+       + A.a[2] * (A.a[3] * A.a[7] - A.a[4] * A.a[6]);
+}
+// Cofactor matrix cof(A) = det(A) A^{-T}.
+template <typename T> MPM_HD M3<T> cofactor(const M3<T>& A) {
+  M3<T> K;
+  K.a[0] = A.a[4] * A.a[8] - A.a[5] * A.a[7];
+  K.a[1] = A.a[5] * A.a[6] - A.a[3] * A.a[8];
+  K.a[2] = A.a[3] * A.a[7] - A.a[4] * A.a[6];
+  K.a[3] = A.a[2] * A.a[7] - A.a[1] * A.a[8];
+  K.a[4] = A.a[0] * A.a[8] - A.a[2] * A.a[6];
+  K.a[5] = A.a[1] * A.a[6] - A.a[0] * A.a[7];
+  K.a[6] = A.a[1] * A.a[5] - A.a[2] * A.a[4];
+  K.a[7] = A.a[2] * A.a[3] - A.a[0] * A.a[5];
+  K.a[8] = A.a[0] * A.a[4] - A.a[1] * A.a[3];
+  return K;
+}
+// A^{-T} (no singularity guard: the reference's np.linalg.inv raises on a
+// singular matrix; here the result is inf/nan and the caller sees it).
+template <typename T> MPM_HD M3<T> inv_t(const M3<T>& A) {
+  M3<T> K = cofactor(A);
+  T id = T(1) / (A.a[0] * K.a[0] + A.a[1] * K.a[1] + A.a[2] * K.a[2]);
+  return scaled(K, id);
+}
+template <typename T> MPM_HD M3<T> inv(const M3<T>& A) { return transpose(inv_t(A)); }
+
+// ---------------------------------------------------------------------------
+// Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (the textbook
+// method, e.g. Golub & Van Loan, "Matrix Computations", sec. 8.5).  On return
+// lam[k] are the eigenvalues and column k of V the matching unit eigenvector,
+// S = V diag(lam) V^T.  Converges quadratically; the sweep count bounds the
+// work and the loop stops early once the off-diagonal part is negligible.
+// ---------------------------------------------------------------------------
+template <typename T> struct JacobiSweeps { static constexpr int value = 6; };
+template <> struct JacobiSweeps<double> { static constexpr int value = 10; };
+
+template <typename T>
+MPM_HD void jacobi_rotate(T* S, M3<T>& V, int p, int q, int r) {
+  // S packed as {s00, s11, s22, s01, s02, s12}; rotation in the (p, q) plane.
+  auto off = [](int i, int j) { return (i + j == 1) ? 3 : (i + j == 2 ? 4 : 5); };
+  const int ipq = off(p, q), ipr = off(p, r), iqr = off(q, r);
+  const T apq = S[ipq];
+  if (apq == T(0)) return;
+  const T app = S[p], aqq = S[q];
+  const T theta = (aqq - app) / (T(2) * apq);
+  const T t = (theta >= T(0) ? T(1) : T(-1)) / (std::fabs(theta) + std::sqrt(T(1) + theta * theta));
+  const T c = T(1) / std::sqrt(T(1) + t * t);
+  const T s = t * c;
+  S[p] = app - t * apq;
+  S[q] = aqq + t * apq;
+  S[ipq] = T(0);
+  const T arp = S[ipr], arq = S[iqr];
+  S[ipr] = c * arp - s * arq;
+  S[iqr] = s * arp + c * arq;
+  for (int k = 0; k < 3; ++k) {
+    const T vkp = V(k, p), vkq = V(k, q);
+    V(k, p) = c * vkp - s * vkq;
+    V(k, q) = s * vkp + c * vkq;
+  }
+}
+
+template <typename T>
+MPM_HD void sym_eig3(const M3<T>& A, T lam[3], M3<T>& V) {
+  T S[6] = {A(0, 0), A(1, 1), A(2, 2), A(0, 1), A(0, 2), A(1, 2)};
+  V = eye3<T>();
+  const T tiny = std::numeric_limits<T>::epsilon() * std::numeric_limits<T>::epsilon();
+  for (int sweep = 0; sweep < JacobiSweeps<T>::value; ++sweep) {
+    const T offn = S[3] * S[3] + S[4] * S[4] + S[5] * S[5];
+    const T diag = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+    if (offn <= tiny * diag) break;
+    jacobi_rotate(S, V, 0, 1, 2);
+    jacobi_rotate(S, V, 0, 2, 1);
+    jacobi_rotate(S, V, 1, 2, 0);
+  }
+  lam[0] = S[0]; lam[1] = S[1]; lam[2] = S[2];
+}
+
+// ---------------------------------------------------------------------------
+// Daleckii-Krein divided difference of log for the Hencky VJP,
+// (ln bi - ln bj) / (bi - bj) with the limit 1/bi at repeated eigenvalues
+// (python/morphompm/constitutive.py:66-71).  FP64 keeps the reference's
+// switch at |bi - bj| <= 1e-9; FP32 uses log1p(r)/r with r = (bi - bj)/bj and
+// its series near r = 0, which has no cancellation.
+// ---------------------------------------------------------------------------
+MPM_HD double dk_logdiff(double bi, double bj, double lbi, double lbj) {
+  const double d = bi - bj;
+  return std::fabs(d) > kDegen ? (lbi - lbj) / d : 1.0 / bi;
+}
+MPM_HD float dk_logdiff(float bi, float bj, float, float) {
+  const float r = (bi - bj) / bj;
+  float q;
+  if (std::fabs(r) < 1e-3f) q = 1.0f - r * (0.5f - r * (1.0f / 3.0f - 0.25f * r));
+  else q = std::log1p(r) / r;
+  return q / bj;
+}
+
+// ---------------------------------------------------------------------------
+// Model parameters: NH/Hencky use prm[0] = mu, prm[1] = lambda; HB uses
+// prm = {kappa, K, n, tau_y, m, eps} (python/morphompm/constitutive.py:115).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ModelParams {
+  int32_t kind;
+  T p[6];
+};
+
+// Hencky core on Fe: tau and the Fe cotangent.  Templated on the compute
+// type so the FP32 path can run the near-repeated branch in FP64.
+template <typename T>
+MPM_HD M3<T> hencky_tau_core(const M3<T>& Fe, T mu, T lam) {
+  T beta[3];
+  M3<T> U;
+  sym_eig3(mul_nt(Fe, Fe), beta, U);
+  T h[3];
+  const T S = T(0.5) * lam * (std::log(beta[0]) + std::log(beta[1]) + std::log(beta[2]));
+  for (int i = 0; i < 3; ++i) h[i] = mu * std::log(beta[i]) + S;
+  M3<T> tau;
+  for (int r = 0; r < 3; ++r)
+    for (int c = r; c < 3; ++c) {
+      const T v = U(r, 0) * h[0] * U(c, 0) + U(r, 1) * h[1] * U(c, 1) + U(r, 2) * h[2] * U(c, 2);
+      tau(r, c) = v;
+      tau(c, r) = v;
+    }
+  return tau;
+}
+
+// Fe cotangent from a precomputed eigen-decomposition b = U diag(beta) U^T.
+template <typename T>
+MPM_HD M3<T> hencky_fe_bar_eig(const M3<T>& Fe, const M3<T>& tb, T mu, T lam, const T beta[3],
+                               const M3<T>& U) {
+  T lb[3];
+  for (int i = 0; i < 3; ++i) lb[i] = std::log(beta[i]);
+  M3<T> Mm = scaled(tb, mu);
+  const T hl = T(0.5) * lam * trace(tb);
+  Mm.a[0] += hl; Mm.a[4] += hl; Mm.a[8] += hl;
+  M3<T> K = mul(mul_tn(U, Mm), U);                       // U^T M U
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) K(i, j) *= dk_logdiff(beta[i], beta[j], lb[i], lb[j]);
+  const M3<T> Bbar = mul_nt(mul(U, K), U);               // U (K o L) U^T
+  return scaled(mul(Bbar, Fe), T(2));                    // b = Fe Fe^T -> 2 Bbar Fe
+}
+
+template <typename D, typename S> MPM_HD M3<D> cast3(const M3<S>& A) {
+  M3<D> B; for (int i = 0; i < 9; ++i) B.a[i] = (D)A.a[i]; return B;
+}
+
+// FP32 near-repeated branch: when the smallest eigenvalue gap of b is below
+// kNearRepeated times tr b, the FP32 path redoes the eigen-decomposition and
+// the divided differences in FP64.
+#ifndef MPM_HENCKY_FP64_DEGEN
+#define MPM_HENCKY_FP64_DEGEN 1
+#endif
+constexpr float kNearRepeated = 1.0e-3f;
+
+MPM_HD bool near_repeated(const float beta[3]) {
+  const float s = std::fabs(beta[0]) + std::fabs(beta[1]) + std::fabs(beta[2]);
+  const float g01 = std::fabs(beta[0] - beta[1]), g02 = std::fabs(beta[0] - beta[2]),
+              g12 = std::fabs(beta[1] - beta[2]);
+  const float g = g01 < g02 ? (g01 < g12 ? g01 : g12) : (g02 < g12 ? g02 : g12);
+  return g < kNearRepeated * s;
+}
+
+MPM_HD M3<double> hencky_fe_bar(const M3<double>& Fe, const M3<double>& tb, double mu, double lam) {
+  double beta[3];
+  M3<double> U;
+  sym_eig3(mul_nt(Fe, Fe), beta, U);
+  return hencky_fe_bar_eig(Fe, tb, mu, lam, beta, U);
+}
+MPM_HD M3<float> hencky_fe_bar(const M3<float>& Fe, const M3<float>& tb, float mu, float lam) {
+  float beta[3];
+  M3<float> U;
+  sym_eig3(mul_nt(Fe, Fe), beta, U);
+#if MPM_HENCKY_FP64_DEGEN
+  if (near_repeated(beta)) {
+    const M3<double> Fed = cast3<double>(Fe);
+    double bd[3];
+    M3<double> Ud;
+    sym_eig3(mul_nt(Fed, Fed), bd, Ud);
+    return cast3<float>(hencky_fe_bar_eig(Fed, cast3<double>(tb), (double)mu, (double)lam, bd, Ud));
+  }
+#endif
+  return hencky_fe_bar_eig(Fe, tb, mu, lam, beta, U);
+}
+
+MPM_HD M3<double> hencky_tau(const M3<double>& Fe, double mu, double lam) {
+  return hencky_tau_core(Fe, mu, lam);
+}
+MPM_HD M3<float> hencky_tau(const M3<float>& Fe, float mu, float lam) {
+  // tau is smooth through repeated eigenvalues: FP32 throughout.
+  return hencky_tau_core(Fe, mu, lam);
+}
+
+// Elastic stress tau(F, Fg) for NH / Hencky; HB handled separately.
+template <typename T>
+MPM_HD M3<T> kirchhoff(const ModelParams<T>& mp, const M3<T>& F, const M3<T>& Fg, const M3<T>& C) {
+  if (mp.kind == HERSCHEL_BULKLEY) {
+    // tau = kappa ln(det F / det Fg) I + 2 eta(gdot) D  (constitutive.py:126-132)
+    const T kappa = mp.p[0], K = mp.p[1], n = mp.p[2], ty = mp.p[3], m = mp.p[4], eps = mp.p[5];
+    M3<T> D = sym(C);
+    const T gd = std::sqrt(T(2) * frob(D, D) + eps * eps);
+    const T one_m_e = -std::expm1(-m * gd);
+    const T eta = K * std::pow(gd, n - T(1)) + ty * one_m_e / gd;
+    const T vol = kappa * std::log(det(F) / det(Fg));
+    M3<T> tau = scaled(D, T(2) * eta);
+    tau.a[0] += vol; tau.a[4] += vol; tau.a[8] += vol;
+    return tau;
+  }
+  const T mu = mp.p[0], lam = mp.p[1];
+  const M3<T> Fe = mul(F, inv(Fg));
+  if (mp.kind == NEO_HOOKEAN) {
+    // tau = mu (Fe Fe^T - I) + lam ln(det Fe) I  (constitutive.py:25-28)
+    M3<T> b = mul_nt(Fe, Fe);
+    const T lj = lam * std::log(det(Fe));
+    M3<T> tau;
+    for (int i = 0; i < 9; ++i) tau.a[i] = mu * b.a[i];
+    tau.a[0] += lj - mu; tau.a[4] += lj - mu; tau.a[8] += lj - mu;
+    return tau;
+  }
+  return hencky_tau(Fe, mu, lam);
+}
+
+// HB (1 - e^{-u}) and u e^{-u} - (1 - e^{-u}) with u = m gdot.  The latter
+// cancels for small u; FP32 uses its Taylor series -u^2/2 + u^3/3 - u^4/8 +
+// u^5/30 - u^6/144 + u^7/840 there.
+MPM_HD double hb_dnum(double u) { const double e = std::exp(-u); return u * e - (1.0 - e); }
+MPM_HD float hb_dnum(float u) {
+  if (u < 0.05f)
+    return u * u * (-0.5f + u * (1.0f / 3.0f + u * (-0.125f + u * (1.0f / 30.0f + u * (-1.0f / 144.0f + u * (1.0f / 840.0f))))));
+  const float e = std::exp(-u);
+  return u * e + std::expm1(-u);
+}
+
+// VJP of tau w.r.t. (F, Fg, C) given tau_bar; theta_bar[0..1] get the HB
+// (tau_y, K) cotangents (zero for the elastic models).
+// NH: constitutive.py:31-39.  Hencky: constitutive.py:50-75.  HB: :134-147.
+template <typename T>
+MPM_HD void kirchhoff_vjp(const ModelParams<T>& mp, const M3<T>& F, const M3<T>& Fg, const M3<T>& C,
+                          const M3<T>& tau_bar, M3<T>& F_bar, M3<T>& Fg_bar, M3<T>& C_bar,
+                          T theta_bar[2]) {
+  const M3<T> tb = sym(tau_bar);
+  const T trtb = trace(tb);
+  theta_bar[0] = T(0);
+  theta_bar[1] = T(0);
+  if (mp.kind == HERSCHEL_BULKLEY) {
+    const T kappa = mp.p[0], K = mp.p[1], n = mp.p[2], ty = mp.p[3], m = mp.p[4], eps = mp.p[5];
+    F_bar = scaled(inv_t(F), kappa * trtb);
+    Fg_bar = scaled(inv_t(Fg), -kappa * trtb);
+    M3<T> D = sym(C);
+    const T gd = std::sqrt(T(2) * frob(D, D) + eps * eps);
+    const T one_m_e = -std::expm1(-m * gd);
+    const T gpow = std::pow(gd, n - T(1));
+    const T eta = K * gpow + ty * one_m_e / gd;
+    const T deta = K * (n - T(1)) * gpow / gd + ty * hb_dnum(m * gd) / (gd * gd);
+    const T tD = frob(tb, D);
+    const T cD = T(4) * tD * deta / gd;
+    for (int i = 0; i < 9; ++i) C_bar.a[i] = T(2) * eta * tb.a[i] + cD * D.a[i];
+    theta_bar[0] = T(2) * one_m_e / gd * tD;
+    theta_bar[1] = T(2) * gpow * tD;
+    return;
+  }
+  C_bar = zero3<T>();
+  const T mu = mp.p[0], lam = mp.p[1];
+  const M3<T> Gi = inv(Fg);
+  const M3<T> Fe = mul(F, Gi);
+  M3<T> Fe_bar;
+  if (mp.kind == NEO_HOOKEAN) {
+    const M3<T> tbFe = mul(tb, Fe);
+    const M3<T> FeiT = inv_t(Fe);
+    for (int i = 0; i < 9; ++i) Fe_bar.a[i] = T(2) * mu * tbFe.a[i] + lam * trtb * FeiT.a[i];
+  } else {
+    Fe_bar = hencky_fe_bar(Fe, tb, mu, lam);
+  }
+  F_bar = mul_nt(Fe_bar, Gi);                              // Fe_bar Gi^T
+  // Fg_bar = -Gi^T F^T Fe_bar Gi^T
+  Fg_bar = scaled(mul_nt(mul_tn(Gi, mul_tn(F, Fe_bar)), Gi), T(-1));
+}
+
+// ---------------------------------------------------------------------------
+// Quadratic B-spline on a node-centred grid (python/morphompm/transfer.py:18-30).
+// base = floor(x/dx - 1/2), fx = x/dx - base in [0.5, 1.5).
+// ---------------------------------------------------------------------------
+// Contraction is off here so every kernel (sort key, payload, gathers)
+// rounds x/dx and x/dx - 1/2 the same way and agrees on the cell.
+template <typename T>
+MPM_HD int cell_base(T x, T inv_dx, T& fx) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  const T X = x * inv_dx;
+  const T b = std::floor(X - T(0.5));
+  fx = X - b;
+  return (int)b;
+}
+template <typename T>
+MPM_HD void bspline_w(T fx, T w[3]) {
+  const T a = T(1.5) - fx, c = fx - T(1), d = fx - T(0.5);
+  w[0] = T(0.5) * a * a;
+  w[1] = T(0.75) - c * c;
+  w[2] = T(0.5) * d * d;
+}
+template <typename T>
+MPM_HD void bspline_dw(T fx, T dw[3]) {
+  dw[0] = fx - T(1.5);
+  dw[1] = T(-2) * (fx - T(1));
+  dw[2] = fx - T(0.5);
+}
+
+}  // namespace mpm
