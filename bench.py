@@ -1,0 +1,302 @@
+"""Forward + adjoint particle-steps/s of the batched growth inversion on one MI355X.
+
+Workload (default): 256 independent free-swell / bilayer instances, particle
+counts log-uniform in [1k, 32k] (seeded), growth rates U(0.02, 0.2), 200 steps
+each, adjoint with checkpoints every 20 steps, FP32.  One bench step = one
+Gauss-Newton iteration of the batched inversion: the forward trajectory and the
+checkpointed adjoint (recompute included) of every instance, captured as HIP
+graphs and replayed, plus the per-instance update on the host.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  Besides the metric it reports the dominant kernel's
+HBM roofline fraction (timed with HIP events inside the graph replays of the
+timed region), the plain single-instance GPU rate (same kernels, one instance
+at a time, no graph), FP32 error vs an FP64 GPU run, and the FP64 numpy
+oracle timed on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("fwd+adjoint particle-steps/s, batched FP32 vs plain GPU, grad rel.err≤1e-5")
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--sim-steps", type=int, default=200)
+    ap.add_argument("--checkpoint-every", type=int, default=20)
+    ap.add_argument("--plain-instances", type=int, default=6)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--tile", default="4,8,8")
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--timer", choices=("graph", "eager"), default="eager",
+                    help="time the dominant kernel inside the captured graph replays of the "
+                         "timed region, or in an eager instrumented iteration after it")
+    return ap.parse_args()
+
+
+# ----------------------------------------------------------------- CPU leg
+def _cpu_worker(args):
+    """One single-thread process: FP64 numpy oracle forward + adjoint of one
+    instance for a truncated number of steps."""
+    os.environ["OMP_NUM_THREADS"] = "1"
+    import numpy as np
+    from oracle import ref_vec as RV
+    from oracle import ref_loop as RL
+    pts, G, dx, dt, steps, sched = args
+    mat = RL.Material(E=1.0e3, nu=0.3, density=1.0)
+    hi = np.ceil(pts.max(0) / dx).astype(int) + 4
+    cfg = RL.SimConfig(N=int(hi.max()), dx=dx, dt=dt, sp=dx / 2, material=mat,
+                       dims=tuple(int(h) for h in hi))
+    st0 = RL.ParticleState.rest(pts)
+    st0.Fg = G.copy()
+    t0 = time.perf_counter()
+    fin, caches = RV.rollout(st0, RL.NeoHookean(mat), cfg, steps, advect=True, sched=sched)
+    n = len(pts)
+    Fb = np.linalg.det(fin.F)[:, None, None] * np.swapaxes(np.linalg.inv(fin.F), 1, 2)
+    RV.rollout_vjp(caches, np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3, 3)), Fb,
+                   np.zeros((n, 3, 3)), advect=True, sched=sched)
+    return n * steps, time.perf_counter() - t0
+
+
+def cpu_baseline(scenes, budget_s):
+    """FP64 vectorised numpy oracle on the host: P single-thread processes over
+    independent instances, each run for a truncated step count."""
+    import multiprocessing as mp
+    P = max(1, min(16, os.cpu_count() or 1))
+    steps = 4
+    jobs = []
+    for s in scenes[: 4 * P]:
+        jobs.append((s.pts, s.G, s.cfg.dx, s.cfg.dt, steps,
+                     s.sched[:steps] if s.sched is not None else None))
+    # calibrate the number of jobs to the time budget with one job
+    n0, t0 = _cpu_worker(jobs[0])
+    per_ps = t0 / n0
+    total_ps = sum(len(j[0]) * steps for j in jobs)
+    while len(jobs) > P and total_ps * per_ps / P > budget_s:
+        jobs.pop()
+        total_ps = sum(len(j[0]) * steps for j in jobs)
+    ctx = mp.get_context("spawn")
+    t = time.perf_counter()
+    with ctx.Pool(P) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    wall = time.perf_counter() - t
+    done = sum(r[0] for r in res)
+    return {"value": done / wall, "unit": "particle-steps/s", "cores": P, "kind": "port",
+            "sample": f"{len(jobs)} of the instances, {steps} fwd+adjoint steps each "
+                      f"(FP64 vectorised numpy oracle, 1 thread per process, {P} processes, "
+                      f"{done} particle-steps in {wall:.1f} s)",
+            "single_thread_value": n0 / t0}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "requested_gpus": a.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from morphompm_amd import _lib
+    from morphompm_amd.batched import BatchedInverse
+    from morphompm_amd.engine import BatchPlan, InstanceSpec, Trajectory, ROW_F
+    from morphompm_amd.scenes import batch_instances
+    import ctypes
+
+    _lib.require_gpu()
+    tile = tuple(int(t) for t in a.tile.split(","))
+    scenes = batch_instances(count=a.instances, n_steps=a.sim_steps)
+    for s in scenes:
+        s.checkpoint_every = a.checkpoint_every
+    n_total = sum(s.n for s in scenes)
+    rates = np.array([s.extra["rate"] for s in scenes])
+    print(f"[bench] {len(scenes)} instances, {n_total} particles, {a.sim_steps} steps, "
+          f"K={a.checkpoint_every}", file=sys.stderr, flush=True)
+
+    bi = BatchedInverse(scenes, dtype=torch.float32, tile=tile, block=a.block, use_graphs=True)
+    # targets: the loss at the true rates (the observed shapes)
+    bi.set_theta(rates)
+    target, _ = bi.loss_and_grad()
+    target = target.cpu().numpy()
+    theta = 0.8 * rates
+
+    # dominant kernel timer, armed for the capture of the adjoint graph
+    kid = 8  # MPM_K_P2G_ADJ
+    n_launch = a.sim_steps
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * n_launch)]
+    for e in evs:
+        e.record()
+    torch.cuda.synchronize()
+    handles = (ctypes.c_void_p * (2 * n_launch))(*[e.cuda_event for e in evs])
+    L = _lib.lib()
+    L.mpm_kernel_timer.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]
+    bi._g_fwd = bi._g_bwd = None
+
+    def capture_with_timer():
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            bi._fwd()
+            bi._bwd()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        bi._g_fwd = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(bi._g_fwd):
+            bi._fwd()
+        bi._g_bwd = torch.cuda.CUDAGraph()
+        L.mpm_kernel_timer(kid, handles, n_launch)
+        with torch.cuda.graph(bi._g_bwd):
+            bi._bwd()
+        L.mpm_kernel_timer(-1, None, 0)
+    if a.timer == "graph":
+        bi._capture = capture_with_timer
+
+    def gn_iteration(theta):
+        bi.set_theta(theta)
+        Lv, dL = (t.cpu().numpy() for t in bi.loss_and_grad())
+        resid = Lv - target
+        ok = np.isfinite(dL) & (np.abs(dL) >= 1e-12)
+        step = np.where(ok, np.clip(resid / np.where(ok, dL, 1.0), -0.5, 0.5), 0.0)
+        theta = theta - step
+        nudge = ~ok & (np.abs(resid) >= 1e-9)
+        theta = theta + np.where(nudge, 1e-3 * np.where(resid < 0, 1.0, -1.0), 0.0)
+        return theta, Lv, dL
+
+    for _ in range(a.warmup):
+        theta, _, _ = gn_iteration(theta)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(a.steps):
+        theta, Lv, dL = gn_iteration(theta)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = e0.elapsed_time(e1)
+    ms_per = ms_total / a.steps
+    ps_per_iter = n_total * a.sim_steps
+    value = ps_per_iter * a.steps / (ms_total / 1e3)
+    print(f"[bench] {ms_per:.2f} ms per iteration, {value:.4g} particle-steps/s "
+          f"(wall {wall:.2f} s)", file=sys.stderr, flush=True)
+
+    if a.timer == "eager":
+        # one more iteration, eager, with the dominant kernel's launches timed
+        bi.set_theta(theta)
+        bi._fwd()
+        L.mpm_kernel_timer(kid, handles, n_launch)
+        bi._bwd()
+        L.mpm_kernel_timer(-1, None, 0)
+        torch.cuda.synchronize()
+    # dominant kernel: average launch duration over one iteration's launches
+    durs = [evs[2 * i].elapsed_time(evs[2 * i + 1]) for i in range(n_launch)]
+    avg_ms = float(np.mean(durs))
+    grid = bi.traj.seg_grid[0]
+    active = int((grid[0] > 1e-12).sum())
+    bytes_launch = 432.0 * n_total + 28.0 * active
+    achieved = bytes_launch / (avg_ms / 1e3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_p2g_adj_batched.txt")
+    if os.path.exists(tf):
+        for line in open(tf):
+            if line.startswith("traffic_bytes_per_launch"):
+                traffic = float(line.split()[1])
+    roof = {"kernel": "k_p2g_adj (P2G-adjoint gather + constitutive VJP)", "bound": "hbm",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_launch,
+            "launches_timed": n_launch, "timer": a.timer}
+
+    # whole-iteration algorithmic bytes (fwd + adjoint, per particle-step)
+    r = active / n_total
+    b_alg = 660 + 128 * r
+    whole = {"B_alg_per_particle_step": b_alg,
+             "achieved_GBs": b_alg * value / 1e9,
+             "frac": b_alg * value / 1e9 / HBM_PEAK_GBS}
+
+    # plain single-instance path: same kernels, one instance at a time, no graph
+    plain = None
+    if a.plain_instances > 0:
+        idx = np.argsort([s.n for s in scenes])[len(scenes) // 2 - a.plain_instances // 2:][:a.plain_instances]
+        tot_ps, tot_t = 0, 0.0
+        for i in idx:
+            s = scenes[i]
+            p = BatchPlan([InstanceSpec.from_cfg(s.n, s.cfg, s.model)], tile=tile, block=a.block)
+            tr = Trajectory(p, torch.float32, s.n_steps, True, a.checkpoint_every, s.sched)
+            dyn0 = p.pack_dyn([(s.pts, np.zeros((s.n, 3)), np.zeros((s.n, 3, 3)),
+                                np.repeat(np.eye(3)[None], s.n, 0))], torch.float32)
+            G = p.pack_mat([np.eye(3) + theta[i] * s.dG], torch.float32)
+            for rep in range(2):
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                fin = tr.forward(dyn0, G)
+                bar = torch.zeros_like(fin)
+                F = fin[ROW_F:ROW_F + 9].to(torch.float64)
+                from morphompm_amd.batched import cofactor_rows
+                bar[ROW_F:ROW_F + 9] = (cofactor_rows(F)[1] * p.live).float()
+                tr.backward(bar)
+                torch.cuda.synchronize()
+                dtt = time.perf_counter() - t1
+            tot_ps += s.n * s.n_steps
+            tot_t += dtt
+        plain = {"per_instance_value": tot_ps / tot_t, "instances_sampled": int(len(idx)),
+                 "median_n": int(np.median([scenes[i].n for i in idx]))}
+
+    # FP32 vs FP64 on the same batch (one forward + adjoint)
+    acc = None
+    if not a.no_fp64_check:
+        bi.set_theta(theta)
+        L32, g32 = (t.cpu().numpy() for t in bi.loss_and_grad())
+        F32 = bi.final[ROW_F:ROW_F + 9].double().clone()
+        del bi
+        torch.cuda.empty_cache()
+        b64 = BatchedInverse(scenes, dtype=torch.float64, tile=tile, block=a.block,
+                             use_graphs=False)
+        b64.set_theta(theta)
+        L64, g64 = (t.cpu().numpy() for t in b64.loss_and_grad())
+        F64 = b64.final[ROW_F:ROW_F + 9]
+        live = b64.plan.live
+        acc = {"forward_max_abs_dF": float((F32 - F64)[:, live].abs().max()),
+               "loss_max_rel_err": float(np.max(np.abs(L32 - L64) / np.abs(L64))),
+               "grad_max_rel_err": float(np.max(np.abs(g32 - g64) / np.abs(g64))),
+               "grad_median_rel_err": float(np.median(np.abs(g32 - g64) / np.abs(g64)))}
+        del b64
+        torch.cuda.empty_cache()
+
+    cpu = None
+    if not a.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(scenes, a.cpu_seconds)
+
+    out = {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_per,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": "batched inverse growth (Gauss-Newton iteration = 200-step "
+                                  "forward + checkpointed adjoint of every instance)",
+                      "instances": len(scenes), "particles": n_total,
+                      "sim_steps": a.sim_steps, "checkpoint_every": a.checkpoint_every,
+                      "tile": list(tile), "block": a.block, "hip_graphs": True},
+           "roofline": roof, "whole_step_roofline": whole, "cpu_baseline": cpu,
+           "plain_gpu": plain,
+           "batched_over_plain": (value / plain["per_instance_value"]) if plain else None,
+           "batched_over_cpu": (value / cpu["value"]) if cpu else None,
+           "fp32_vs_fp64": acc}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

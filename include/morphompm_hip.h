@@ -140,6 +140,27 @@ int32_t mpm_step_bwd_f64(const mpm_batch* b, const double* dyn_in, const double*
                          double* Fg_bar, double* theta_acc, int32_t advect, void* ws,
                          void* stream);
 
+/* Per-launch timing of one kernel (profiling aid, no reference counterpart).
+ * `events` holds 2*n_pairs hipEvent_t handles created with timing enabled.
+ * Every later launch of kernel `kernel_id` records events[2i] just before and
+ * events[2i+1] just after it on its stream, for the i-th such launch while
+ * i < n_pairs; inside stream capture they become external event-record nodes,
+ * so each graph replay re-records them.  kernel_id < 0 disarms.  Process-wide
+ * state, not thread-safe.  mpm_kernel_timer_count() = launches recorded. */
+enum {
+  MPM_K_SORT = 0,        /* the whole counting-sort sequence */
+  MPM_K_PREP_FWD = 1,
+  MPM_K_P2G_FWD = 2,     /* forward P2G tile reduction */
+  MPM_K_GRID_FWD = 3,
+  MPM_K_G2P = 4,
+  MPM_K_PREP_ADJ = 5,
+  MPM_K_P2G_ADJ_TILE = 6, /* G2P-adjoint scatter as a tile reduction */
+  MPM_K_GRID_ADJ = 7,
+  MPM_K_P2G_ADJ = 8      /* P2G-adjoint gather + constitutive VJP */
+};
+int32_t mpm_kernel_timer(int32_t kernel_id, void* const* events, int32_t n_pairs);
+int32_t mpm_kernel_timer_count(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -763,6 +763,40 @@ int validate(const mpm_batch* b) {
 
 inline dim3 grid1(int64_t n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
 
+// Kernel timer (mpm_kernel_timer): events around the armed kernel's launches.
+struct KernelTimer {
+  int id = -1;
+  hipEvent_t* ev = nullptr;
+  int n_pairs = 0;
+  int count = 0;
+};
+KernelTimer g_timer;
+
+void record_event(hipEvent_t e, hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive)
+    (void)hipEventRecordWithFlags(e, st, hipEventRecordExternal);
+  else
+    (void)hipEventRecord(e, st);
+}
+
+struct TimedLaunch {
+  bool on;
+  hipStream_t st;
+  hipEvent_t stop;
+  TimedLaunch(int id, hipStream_t s) : on(false), st(s), stop(nullptr) {
+    if (g_timer.id == id && g_timer.count < g_timer.n_pairs) {
+      on = true;
+      record_event(g_timer.ev[2 * g_timer.count], st);
+      stop = g_timer.ev[2 * g_timer.count + 1];
+      ++g_timer.count;
+    }
+  }
+  ~TimedLaunch() {
+    if (on) record_event(stop, st);
+  }
+};
+
 template <typename T>
 int sort_impl(const mpm_batch* b, const T* dyn, int32_t* sortbuf, const Workspace& w,
               hipStream_t st) {
@@ -770,6 +804,7 @@ int sort_impl(const mpm_batch* b, const T* dyn, int32_t* sortbuf, const Workspac
   int32_t* sorted = sortbuf + b->n_slots;
   int32_t* start = sortbuf + 2 * b->n_slots;
   const int bs = b->block;
+  TimedLaunch tl(MPM_K_SORT, st);
   k_zero_i32<<<grid1(b->n_bins, 256), 256, 0, st>>>(w.count, b->n_bins);
   k_sort_key<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn,
                                                        w.key, w.slot, w.count);
@@ -785,8 +820,12 @@ int sort_impl(const mpm_batch* b, const T* dyn, int32_t* sortbuf, const Workspac
 template <typename T, int TX, int TY, int TZ>
 void launch_tile_fwd(const mpm_batch* b, const int32_t* start, const Workspace& w, T* grid,
                      hipStream_t st) {
+  {
+  TimedLaunch tl(MPM_K_P2G_FWD, st);
   k_p2g_tile<T, TX, TY, TZ, true><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
       b->inst, b->tile_inst, start, (const T*)w.payload, b->n_slots, (T*)w.partial);
+  }
+  TimedLaunch tl(MPM_K_GRID_FWD, st);
   k_grid_fwd<T, TX, TY, TZ><<<grid1(b->n_nodes, 256), 256, 0, st>>>(
       b->inst, b->node_inst, b->n_nodes, (const T*)w.partial, grid);
 }
@@ -794,8 +833,12 @@ void launch_tile_fwd(const mpm_batch* b, const int32_t* start, const Workspace& 
 template <typename T, int TX, int TY, int TZ>
 void launch_tile_adj(const mpm_batch* b, const int32_t* start, const Workspace& w, const T* grid,
                      int advect, hipStream_t st) {
+  {
+  TimedLaunch tl(MPM_K_P2G_ADJ_TILE, st);
   k_p2g_tile<T, TX, TY, TZ, false><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
       b->inst, b->tile_inst, start, (const T*)w.payload, b->n_slots, (T*)w.partial);
+  }
+  TimedLaunch tl(MPM_K_GRID_ADJ, st);
   k_grid_adj<T, TX, TY, TZ><<<grid1(b->n_nodes, 256), 256, 0, st>>>(
       b->inst, b->node_inst, b->n_nodes, (const T*)w.partial, grid, (T*)w.gadj, advect);
 }
@@ -838,9 +881,13 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   const int32_t* rank = sortbuf;
   const int32_t* start = sortbuf + 2 * b->n_slots;
   const int bs = b->block;
+  {
+  TimedLaunch tl(MPM_K_PREP_FWD, st);
   k_prep_fwd<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in,
                                                       Fg, sched, rank, (T*)w.payload);
+  }
   dispatch_fwd<T>(tile_index(b), b, start, w, grid, st);
+  TimedLaunch tl(MPM_K_G2P, st);
   k_g2p<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, b->n_nodes,
                                                  dyn_in, grid, dyn_out, advect);
   return check_launch("step_fwd");
@@ -861,9 +908,13 @@ int step_bwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   const int32_t* rank = sortbuf;
   const int32_t* start = sortbuf + 2 * b->n_slots;
   const int bs = b->block;
+  {
+  TimedLaunch tl(MPM_K_PREP_ADJ, st);
   k_prep_adj<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in,
                                                       bar_out, rank, (T*)w.payload, advect);
+  }
   dispatch_adj<T>(tile_index(b), b, start, w, grid, advect, st);
+  TimedLaunch tl(MPM_K_P2G_ADJ, st);
   k_p2g_adj<T><<<grid1(b->n_slots, 256), 256, 0, st>>>(
       b->inst, b->slot_inst, b->n_slots, b->n_nodes, dyn_in, Fg, sched, dyn_out, grid,
       (const T*)w.gadj, bar_out, bar_in, Fg_bar, theta_acc, advect);
@@ -907,6 +958,17 @@ int stress_vjp_impl(int32_t model, const double* prm6, int64_t n, int64_t stride
 extern "C" {
 
 int32_t mpm_abi_version(void) { return MPM_ABI_VERSION; }
+
+int32_t mpm_kernel_timer(int32_t kernel_id, void* const* events, int32_t n_pairs) {
+  if (kernel_id >= 0 && (events == nullptr || n_pairs <= 0))
+    return set_err(MPM_EINVAL, "kernel_timer: events required");
+  g_timer.id = kernel_id;
+  g_timer.ev = (hipEvent_t*)events;
+  g_timer.n_pairs = kernel_id >= 0 ? n_pairs : 0;
+  g_timer.count = 0;
+  return MPM_OK;
+}
+int32_t mpm_kernel_timer_count(void) { return g_timer.count; }
 const char* mpm_last_error(void) { return g_err; }
 
 int64_t mpm_workspace_bytes(const mpm_batch* b, int32_t dtype) {

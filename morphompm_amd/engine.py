@@ -1,0 +1,340 @@
+"""Batch layout and the checkpointed trajectory engine over the HIP kernels.
+
+A ``BatchPlan`` lays out one or more independent simulations ("instances")
+for single launches: every instance's particle slots and grid nodes start at
+a multiple of 256 so each 256-slot chunk belongs to one instance, bins and
+P2G tiles are numbered instance after instance, and the instance table plus
+the chunk->instance maps live in device memory.
+
+A ``Trajectory`` runs n_steps forward steps and their reverse.  It stores a
+checkpoint of the particle state every K steps; the reverse recomputes one
+segment of K steps at a time (keeping each step's state, grid and sort
+record), then runs the reverse steps of that segment.  Memory is bounded by
+(n_steps / K) checkpoints plus one segment.  The last segment of the forward
+pass is kept, so it is not recomputed.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import MpmBatch, MpmInstance, SLOT_ALIGN, check, ptr, stream_ptr, suffix
+
+N_DYN = 24        # x(3) v(3) C(9) F(9)
+ROW_X, ROW_V, ROW_C, ROW_F = 0, 3, 6, 15
+DEFAULT_TILE = (4, 8, 8)
+
+
+def model_spec(model):
+    """(kind, prm6) of a constitutive model object; NotImplementedError if the
+    model has no device implementation (there is no CPU fallback)."""
+    kind = getattr(model, "kind", None)
+    if kind in ("nh", "hencky"):
+        return (_lib.NEO_HOOKEAN if kind == "nh" else _lib.HENCKY,
+                (float(model.mu), float(model.lam), 0.0, 0.0, 0.0, 0.0))
+    if kind == "hb":
+        return (_lib.HERSCHEL_BULKLEY,
+                (float(model.kappa), float(model.K), float(model.n), float(model.tau_y),
+                 float(model.m), float(model.eps)))
+    raise NotImplementedError(f"model {type(model).__name__} has no HIP implementation")
+
+
+@dataclass
+class InstanceSpec:
+    n: int
+    shape: tuple
+    dx: float
+    dt: float
+    damping: float
+    pmass: float
+    vol0: float
+    kind: int
+    prm: tuple
+
+    @staticmethod
+    def from_cfg(n, cfg, model):
+        kind, prm = model_spec(model)
+        return InstanceSpec(int(n), tuple(cfg.shape), float(cfg.dx), float(cfg.dt),
+                            float(cfg.damping), float(cfg.pmass), float(cfg.vol0), kind, prm)
+
+
+def _ceil(a, b):
+    return -(-a // b)
+
+
+class BatchPlan:
+    def __init__(self, specs, tile=DEFAULT_TILE, block=256, device="cuda"):
+        _lib.require_gpu()
+        self.specs = list(specs)
+        self.tile = tuple(int(t) for t in tile)
+        self.block = int(block)
+        self.device = torch.device(device)
+        if self.tile not in _lib.tile_shapes():
+            raise ValueError(f"tile {self.tile} not built; available {_lib.tile_shapes()}")
+        insts = (MpmInstance * len(self.specs))()
+        slot_inst, node_inst, tile_inst = [], [], []
+        p_off = g_off = c_off = t_off = 0
+        self.p_off, self.g_off = [], []
+        for i, s in enumerate(self.specs):
+            nx, ny, nz = s.shape
+            tx, ty, tz = (_ceil(nx + 2, self.tile[0]), _ceil(ny + 2, self.tile[1]),
+                          _ceil(nz + 2, self.tile[2]))
+            I = insts[i]
+            I.p_off, I.g_off, I.c_off, I.t_off = p_off, g_off, c_off, t_off
+            I.n, I.nx, I.ny, I.nz, I.tx, I.ty, I.tz = s.n, nx, ny, nz, tx, ty, tz
+            I.model = s.kind
+            I.dx, I.dt, I.damping, I.pmass, I.vol0 = s.dx, s.dt, s.damping, s.pmass, s.vol0
+            for k in range(6):
+                I.prm[k] = s.prm[k]
+            self.p_off.append(p_off)
+            self.g_off.append(g_off)
+            ps = max(1, _ceil(s.n, SLOT_ALIGN)) * SLOT_ALIGN
+            gs = _ceil(nx * ny * nz, SLOT_ALIGN) * SLOT_ALIGN
+            slot_inst += [i] * (ps // SLOT_ALIGN)
+            node_inst += [i] * (gs // SLOT_ALIGN)
+            tile_inst += [i] * (tx * ty * tz)
+            p_off += ps
+            g_off += gs
+            c_off += (nx + 2) * (ny + 2) * (nz + 2) + 1
+            t_off += tx * ty * tz
+        self.n_slots, self.n_nodes, self.n_bins, self.n_tiles = p_off, g_off, c_off, t_off
+        self.n_inst = len(self.specs)
+        self._insts = insts
+        raw = bytes(insts)
+        self.d_inst = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
+        self.d_slot_inst = torch.tensor(slot_inst, dtype=torch.int32, device=self.device)
+        self.d_node_inst = torch.tensor(node_inst, dtype=torch.int32, device=self.device)
+        self.d_tile_inst = torch.tensor(tile_inst, dtype=torch.int32, device=self.device)
+        c = MpmBatch()
+        c.n_inst = self.n_inst
+        c.tile[0], c.tile[1], c.tile[2] = self.tile
+        c.n_slots, c.n_nodes, c.n_bins, c.n_tiles = (self.n_slots, self.n_nodes, self.n_bins,
+                                                     self.n_tiles)
+        c.block = self.block
+        c.inst = self.d_inst.data_ptr()
+        c.slot_inst = self.d_slot_inst.data_ptr()
+        c.node_inst = self.d_node_inst.data_ptr()
+        c.tile_inst = self.d_tile_inst.data_ptr()
+        self.c = c
+        self.sortlen = 2 * self.n_slots + self.n_bins + 1
+        self._ws = {}
+        # slot -> live mask and per-chunk instance (host copies for packing)
+        self.slot_inst_np = np.repeat(np.array(slot_inst, dtype=np.int64), SLOT_ALIGN)
+        live = np.zeros(self.n_slots, dtype=bool)
+        for i, s in enumerate(self.specs):
+            live[self.p_off[i]:self.p_off[i] + s.n] = True
+        self.live = torch.from_numpy(live).to(self.device)
+
+    # ------------------------------------------------------------ buffers
+    def workspace(self, dtype):
+        if dtype not in self._ws:
+            nb = _lib.lib().mpm_workspace_bytes(ctypes.byref(self.c), 1 if dtype == torch.float64 else 0)
+            if nb < 0:
+                check(int(nb), "workspace_bytes")
+            self._ws[dtype] = torch.zeros(int(nb), dtype=torch.uint8, device=self.device)
+        return self._ws[dtype]
+
+    def new_dyn(self, dtype):
+        return torch.zeros(N_DYN, self.n_slots, dtype=dtype, device=self.device)
+
+    def new_mat(self, dtype):
+        return torch.zeros(9, self.n_slots, dtype=dtype, device=self.device)
+
+    def new_grid(self, dtype):
+        return torch.zeros(4, self.n_nodes, dtype=dtype, device=self.device)
+
+    def new_sort(self):
+        return torch.zeros(self.sortlen, dtype=torch.int32, device=self.device)
+
+    # ------------------------------------------------------------ packing
+    def sl(self, i):
+        return slice(self.p_off[i], self.p_off[i] + self.specs[i].n)
+
+    def pack_dyn(self, parts, dtype):
+        """parts: list over instances of (x, v, C, F) with shapes (n,3),(n,3),(n,3,3),(n,3,3)."""
+        dyn = self.new_dyn(dtype)
+        for i, (x, v, C, F) in enumerate(parts):
+            s = self.sl(i)
+            n = self.specs[i].n
+            dyn[ROW_X:ROW_X + 3, s] = _dev(x, dtype, self.device).reshape(n, 3).T
+            dyn[ROW_V:ROW_V + 3, s] = _dev(v, dtype, self.device).reshape(n, 3).T
+            dyn[ROW_C:ROW_C + 9, s] = _dev(C, dtype, self.device).reshape(n, 9).T
+            dyn[ROW_F:ROW_F + 9, s] = _dev(F, dtype, self.device).reshape(n, 9).T
+        return dyn
+
+    def pack_mat(self, mats, dtype):
+        out = self.new_mat(dtype)
+        for i, M in enumerate(mats):
+            n = self.specs[i].n
+            out[:, self.sl(i)] = _dev(M, dtype, self.device).reshape(n, 9).T
+        return out
+
+    def unpack_dyn(self, dyn, i=0):
+        s = self.sl(i)
+        n = self.specs[i].n
+        return (dyn[ROW_X:ROW_X + 3, s].T, dyn[ROW_V:ROW_V + 3, s].T,
+                dyn[ROW_C:ROW_C + 9, s].T.reshape(n, 3, 3), dyn[ROW_F:ROW_F + 9, s].T.reshape(n, 3, 3))
+
+    def unpack_mat(self, M, i=0):
+        n = self.specs[i].n
+        return M[:, self.sl(i)].T.reshape(n, 3, 3)
+
+    def per_instance_sum(self, v):
+        """Sum of a per-slot float64 vector over each instance's live slots, in a
+        fixed order (cumulative sum over slots), -> (n_inst,)."""
+        v = torch.where(self.live, v.to(torch.float64), torch.zeros((), dtype=torch.float64,
+                                                                      device=self.device))
+        cs = torch.cumsum(torch.cat([v.new_zeros(1), v]), 0)
+        starts = torch.tensor(self.p_off, device=self.device)
+        ends = starts + torch.tensor([s.n for s in self.specs], device=self.device)
+        return cs[ends] - cs[starts]
+
+    # ------------------------------------------------------------ kernels
+    def step_fwd(self, dyn_in, G, sched_row, dyn_out, grid, sortbuf, advect, stream=None):
+        f = getattr(_lib.lib(), f"mpm_step_fwd_{suffix(dyn_in.dtype)}")
+        rc = f(ctypes.byref(self.c), ptr(dyn_in), ptr(G), sched_row, ptr(dyn_out), ptr(grid),
+               ptr(sortbuf), int(bool(advect)), ptr(self.workspace(dyn_in.dtype)),
+               stream_ptr(stream))
+        check(rc, "step_fwd")
+
+    def step_bwd(self, dyn_in, G, sched_row, dyn_out, grid, sortbuf, bar_out, bar_in, G_bar,
+                 theta_acc, advect, stream=None):
+        f = getattr(_lib.lib(), f"mpm_step_bwd_{suffix(dyn_in.dtype)}")
+        rc = f(ctypes.byref(self.c), ptr(dyn_in), ptr(G), sched_row, ptr(dyn_out), ptr(grid),
+               ptr(sortbuf), ptr(bar_out), ptr(bar_in), ptr(G_bar), ptr(theta_acc),
+               int(bool(advect)), ptr(self.workspace(dyn_in.dtype)), stream_ptr(stream))
+        check(rc, "step_bwd")
+
+    def sort_by_cell(self, dyn, sortbuf, stream=None):
+        f = getattr(_lib.lib(), f"mpm_sort_by_cell_{suffix(dyn.dtype)}")
+        check(f(ctypes.byref(self.c), ptr(dyn), ptr(sortbuf), ptr(self.workspace(dyn.dtype)),
+                stream_ptr(stream)), "sort_by_cell")
+
+    def theta_per_instance(self, theta_acc):
+        """theta_acc [n_slots/256, 2] float64 -> (n_inst, 2) in chunk order."""
+        chunk_inst = torch.from_numpy(self.slot_inst_np[::SLOT_ALIGN].copy()).to(self.device)
+        out = torch.zeros(self.n_inst, 2, dtype=torch.float64, device=self.device)
+        cs = torch.cumsum(torch.cat([theta_acc.new_zeros(1, 2), theta_acc]), 0)
+        counts = torch.bincount(chunk_inst, minlength=self.n_inst)
+        ends = torch.cumsum(counts, 0)
+        starts = ends - counts
+        out[:] = cs[ends] - cs[starts]
+        return out
+
+
+def _dev(a, dtype, device):
+    if isinstance(a, torch.Tensor):
+        return a.to(device=device, dtype=dtype)
+    return torch.as_tensor(np.asarray(a), dtype=dtype, device=device)
+
+
+class Trajectory:
+    """n_steps forward steps with checkpoints every K steps, and their reverse.
+
+    ``sched``: None (growth tensor used as is, the reference semantics) or a
+    (n_steps, n_inst) tensor a_t: step t uses Fg = I + a_t (G - I).
+    """
+
+    def __init__(self, plan, dtype, n_steps, advect, checkpoint_every=None, sched=None,
+                 stream=None):
+        self.plan = plan
+        self.dtype = dtype
+        self.n_steps = int(n_steps)
+        self.advect = bool(advect)
+        K = int(checkpoint_every) if checkpoint_every else self.n_steps
+        self.K = max(1, min(K, max(1, self.n_steps)))
+        self.nseg = _ceil(self.n_steps, self.K) if self.n_steps > 0 else 0
+        self.stream = stream
+        if sched is not None:
+            sched = torch.as_tensor(sched, dtype=dtype, device=plan.device)
+            if sched.ndim == 1:
+                sched = sched[:, None].expand(self.n_steps, plan.n_inst)
+            sched = sched.contiguous()
+            if sched.shape != (self.n_steps, plan.n_inst):
+                raise ValueError(f"sched shape {tuple(sched.shape)} != ({self.n_steps}, {plan.n_inst})")
+        self.sched = sched
+        p = plan
+        self.ckpt = torch.zeros(max(1, self.nseg), N_DYN, p.n_slots, dtype=dtype, device=p.device)
+        Ks = self.K
+        self.seg_dyn = torch.zeros(Ks, N_DYN, p.n_slots, dtype=dtype, device=p.device)
+        self.seg_grid = torch.zeros(Ks, 4, p.n_nodes, dtype=dtype, device=p.device)
+        self.seg_sort = torch.zeros(Ks, p.sortlen, dtype=torch.int32, device=p.device)
+        self.G = p.new_mat(dtype)
+        self.bar_a = p.new_dyn(dtype)
+        self.bar_b = p.new_dyn(dtype)
+        self.G_bar = p.new_mat(dtype)
+        self.theta_acc = torch.zeros(p.n_slots // SLOT_ALIGN, 2, dtype=torch.float64,
+                                     device=p.device)
+        self._loaded = -1      # segment whose steps sit in seg_*
+
+    def _a(self, t):
+        if self.sched is None:
+            return None
+        row = self.sched[t]
+        return ctypes.c_void_p(row.data_ptr())
+
+    def _seg(self, s):
+        t0 = s * self.K
+        return t0, min(self.K, self.n_steps - t0)
+
+    def _src(self, s, k):
+        return self.ckpt[s] if k == 0 else self.seg_dyn[k - 1]
+
+    def _run_segment(self, s, keep):
+        t0, L = self._seg(s)
+        for k in range(L):
+            dst = self.seg_dyn[k] if (keep or k < L - 1) else self.ckpt[s + 1]
+            self.plan.step_fwd(self._src(s, k), self.G, self._a(t0 + k), dst, self.seg_grid[k],
+                               self.seg_sort[k], self.advect, self.stream)
+        self._loaded = s if keep else -1
+
+    def forward(self, dyn0, G=None):
+        """Runs the trajectory from dyn0 ([24, n_slots]); returns the final state
+        (a view into this trajectory's storage)."""
+        if G is not None:
+            self.G.copy_(G)
+        self.ckpt[0].copy_(dyn0)
+        if self.n_steps == 0:
+            return self.ckpt[0]
+        for s in range(self.nseg):
+            self._run_segment(s, keep=(s == self.nseg - 1))
+        return self.final()
+
+    def final(self):
+        if self.n_steps == 0:
+            return self.ckpt[0]
+        _, L = self._seg(self.nseg - 1)
+        if self._loaded != self.nseg - 1:
+            self._run_segment(self.nseg - 1, keep=True)
+        return self.seg_dyn[L - 1]
+
+    def backward(self, bar_final, G_bar_final=None, want_theta=False):
+        """Cotangents of (dyn0, G) from the cotangent of the final state.
+        Returns (bar0 [24, n_slots], G_bar [9, n_slots], theta [n_inst, 2] or None).
+        Returned tensors are this trajectory's buffers (copy to keep)."""
+        self.bar_a.copy_(bar_final)
+        if G_bar_final is None:
+            self.G_bar.zero_()
+        elif self.sched is None:
+            self.G_bar.copy_(G_bar_final)
+        else:
+            # the final state's Fg is the last step's I + a (G - I)
+            a = self.sched[self.n_steps - 1]
+            chunk = torch.from_numpy(self.plan.slot_inst_np).to(self.plan.device)
+            self.G_bar.copy_(G_bar_final * a[chunk][None, :])
+        self.theta_acc.zero_()
+        bar_out, bar_in = self.bar_a, self.bar_b
+        th = self.theta_acc if want_theta else None
+        for s in range(self.nseg - 1, -1, -1):
+            if self._loaded != s:
+                self._run_segment(s, keep=True)
+            t0, L = self._seg(s)
+            for k in range(L - 1, -1, -1):
+                self.plan.step_bwd(self._src(s, k), self.G, self._a(t0 + k), self.seg_dyn[k],
+                                   self.seg_grid[k], self.seg_sort[k], bar_out, bar_in,
+                                   self.G_bar, th, self.advect, self.stream)
+                bar_out, bar_in = bar_in, bar_out
+        theta = self.plan.theta_per_instance(self.theta_acc) if want_theta else None
+        return bar_out, self.G_bar, theta
