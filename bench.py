@@ -205,7 +205,7 @@ def main():
     durs = [evs[2 * i].elapsed_time(evs[2 * i + 1]) for i in range(n_launch)]
     avg_ms = float(np.mean(durs))
     grid = bi.traj.seg_grid[0]
-    active = int((grid[0] > 1e-12).sum())
+    active = int((grid[:, 0] > 1e-12).sum())
     bytes_launch = 432.0 * n_total + 28.0 * active
     achieved = bytes_launch / (avg_ms / 1e3) / 1e9
     traffic = None
@@ -234,7 +234,8 @@ def main():
         tot_ps, tot_t = 0, 0.0
         for i in idx:
             s = scenes[i]
-            p = BatchPlan([InstanceSpec.from_cfg(s.n, s.cfg, s.model)], tile=tile, block=a.block)
+            p = BatchPlan([InstanceSpec.from_cfg(s.n, s.cfg, s.model, s.pts)], tile=tile,
+                          block=a.block)
             tr = Trajectory(p, torch.float32, s.n_steps, True, a.checkpoint_every, s.sched)
             dyn0 = p.pack_dyn([(s.pts, np.zeros((s.n, 3)), np.zeros((s.n, 3, 3)),
                                 np.repeat(np.eye(3)[None], s.n, 0))], torch.float32)
@@ -260,14 +261,14 @@ def main():
     if not a.no_fp64_check:
         bi.set_theta(theta)
         L32, g32 = (t.cpu().numpy() for t in bi.loss_and_grad())
-        F32 = bi.final[ROW_F:ROW_F + 9].double().clone()
+        F32 = bi.final_state()[ROW_F:ROW_F + 9].double().clone()
         del bi
         torch.cuda.empty_cache()
         b64 = BatchedInverse(scenes, dtype=torch.float64, tile=tile, block=a.block,
                              use_graphs=False)
         b64.set_theta(theta)
         L64, g64 = (t.cpu().numpy() for t in b64.loss_and_grad())
-        F64 = b64.final[ROW_F:ROW_F + 9]
+        F64 = b64.final_state()[ROW_F:ROW_F + 9]
         live = b64.plan.live
         acc = {"forward_max_abs_dF": float((F32 - F64)[:, live].abs().max()),
                "loss_max_rel_err": float(np.max(np.abs(L32 - L64) / np.abs(L64))),

@@ -17,7 +17,7 @@
  *   Fg   [9][n_slots] : the growth tensor G; the step uses
  *                       Fg = I + a (G - I) with a = sched[instance]
  *                       (sched == NULL means a = 1, i.e. Fg = G exactly)
- * Grid layout: grid [4][n_nodes]: row 0 node mass, rows 1-3 node velocity.
+ * Grid layout: grid [n_nodes][4]: per node (mass, vx, vy, vz).
  * Sort record for one step: sortbuf int32 [2 n_slots + n_bins + 1]:
  *   rank[n_slots] | sorted[n_slots] | bin_start[n_bins + 1]
  *
@@ -41,9 +41,15 @@ enum { MPM_OK = 0, MPM_EINVAL = -1, MPM_ELAUNCH = -2, MPM_ENOTSUP = -3 };
 enum { MPM_NEO_HOOKEAN = 0, MPM_HENCKY = 1, MPM_HERSCHEL_BULKLEY = 2 };
 
 /* One independent simulation (python/morphompm/config.py:21-40 SimConfig plus
- * its constitutive model).  Node grid nx*ny*nz covers [0, n*dx) per axis;
- * bins are (nx+2)(ny+2)(nz+2) particle cells keyed by base+2, plus one dead
- * bin for particles with no in-grid node. */
+ * its constitutive model).  Node grid nx*ny*nz covers [0, n*dx) per axis.
+ * Particles are binned by cell base+2 in [0, n+2) per axis; with b' = bin +
+ * shift, the bins are grouped into tiles of tile[0]*tile[1]*tile[2] bins
+ * (tx*ty*tz tiles, tx = ceil((nx+2+shift[0])/tile[0]) ...) and numbered
+ * tile-major:
+ *   key = c_off + ((itx*ty + ity)*tz + itz) * TB + local bin,  itx = b'x / tile[0],
+ *   local bin from b' mod tile, TB = tile volume,
+ * followed by one dead bin (key c_off + tx*ty*tz*TB) for particles with no
+ * in-grid node.  An instance owns tx*ty*tz*TB + 1 bins. */
 typedef struct mpm_instance {
   int64_t p_off; /* first particle slot, multiple of MPM_SLOT_ALIGN */
   int64_t g_off; /* first node, multiple of MPM_SLOT_ALIGN */
@@ -53,6 +59,9 @@ typedef struct mpm_instance {
   int32_t nx, ny, nz;
   int32_t tx, ty, tz; /* P2G tiles per axis */
   int32_t model;      /* MPM_NEO_HOOKEAN | MPM_HENCKY | MPM_HERSCHEL_BULKLEY */
+  int32_t shift[3];   /* tile-grid offset in bins, 0 <= shift < tile: bin b lies in
+                         tile (b + shift) / tile; aligns tiles with occupied cells */
+  int32_t pad;
   double dx, dt, damping, pmass, vol0;
   double prm[6]; /* NH/Hencky: mu, lambda; HB: kappa, K, n, tau_y, m, eps */
 } mpm_instance;
@@ -67,7 +76,8 @@ typedef struct mpm_batch {
   int64_t n_bins;    /* bins of all instances */
   int64_t n_tiles;   /* P2G tiles of all instances */
   int32_t block;     /* threads per block of the particle kernels: 64, 128 or 256 */
-  int32_t reserved;
+  int32_t model_mask; /* bit m set when some instance uses model m; one bit set selects
+                         kernels specialised for that model */
   const mpm_instance* inst;  /* device [n_inst] */
   const int32_t* slot_inst;  /* device [n_slots / 256]: instance owning each 256 slots */
   const int32_t* node_inst;  /* device [n_nodes / 256] */

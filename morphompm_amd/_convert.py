@@ -18,7 +18,7 @@ class Like:
 
     def out(self, t):
         if self.is_torch:
-            return t.to(self.device)
+            return t.to(self.device, copy=True)
         return t.detach().cpu().numpy().copy()
 
 

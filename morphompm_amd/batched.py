@@ -49,7 +49,7 @@ class BatchedInverse:
             raise ValueError("all instances of a batch share the advect flag")
         self.advect = adv.pop()
         self.dtype = dtype
-        specs = [InstanceSpec.from_cfg(s.n, s.cfg, s.model) for s in scenes]
+        specs = [InstanceSpec.from_cfg(s.n, s.cfg, s.model, s.pts) for s in scenes]
         self.plan = p = BatchPlan(specs, tile=tile, block=block)
         K = checkpoint_every or scenes[0].checkpoint_every
         sched = np.stack([s.sched if s.sched is not None else np.ones(self.n_steps)
@@ -57,10 +57,11 @@ class BatchedInverse:
         self.traj = Trajectory(p, dtype, self.n_steps, self.advect, K,
                                None if np.all(sched == 1.0) else sched)
         n = [s.n for s in scenes]
+        self.perm = p.cell_order([s.pts for s in scenes])
         self.dyn0 = p.pack_dyn([(s.pts, np.zeros((k, 3)), np.zeros((k, 3, 3)),
                                  np.repeat(np.eye(3)[None], k, 0)) for s, k in zip(scenes, n)],
-                               dtype)
-        self.dG = p.pack_mat([s.dG for s in scenes], torch.float64)
+                               dtype, self.perm)
+        self.dG = p.pack_mat([s.dG for s in scenes], torch.float64, self.perm)
         self.eye9 = torch.zeros(9, p.n_slots, dtype=torch.float64, device=p.device)
         self.eye9[[0, 4, 8]] = p.live.to(torch.float64)
         self.slot_inst = torch.from_numpy(p.slot_inst_np).to(p.device)
@@ -68,7 +69,7 @@ class BatchedInverse:
         self.bar = p.new_dyn(dtype)
         self.use_graphs = use_graphs
         self._g_fwd = self._g_bwd = None
-        self.final = None
+        self._final = None
 
     @property
     def particles(self):
@@ -81,7 +82,13 @@ class BatchedInverse:
 
     # -- the two phases, each one launch sequence over the whole batch --------
     def _fwd(self):
-        self.final = self.traj.forward(self.dyn0, self.G)
+        self._final = self.traj.forward(self.dyn0, self.G)
+
+    def final_state(self):
+        """The final particle state [24, n_slots] of the last forward run.  The
+        adjoint's recompute reuses the segment store, so this reloads the last
+        segment when needed (read it before the next forward)."""
+        return self.traj.final()
 
     def _bwd(self):
         self.traj.backward(self.bar)
@@ -108,7 +115,7 @@ class BatchedInverse:
             self._g_fwd.replay()
         else:
             self._fwd()
-        return self.final
+        return self._final
 
     def run_backward(self):
         if self.use_graphs:

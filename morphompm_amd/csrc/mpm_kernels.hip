@@ -25,6 +25,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 
 #include "../../include/morphompm_hip.h"
 #include "mpm_math.h"
@@ -35,6 +36,7 @@ namespace {
 
 constexpr int kChunk = MPM_SLOT_ALIGN;  // slots per instance-table entry
 constexpr int kPayload = 15;            // fx[3], Q[3], A'[9]
+constexpr int kRecord = 16;             // payload record, padded to 4 quads
 constexpr int kScanBlock = 256;
 constexpr int kScanPer = 4;             // elements per thread in the bin scan
 
@@ -58,17 +60,20 @@ int check_launch(const char* what) {
 template <typename T>
 struct Inst {
   int64_t p_off, g_off, c_off, t_off;
-  int n, nx, ny, nz, tx, ty, tz;
+  int n, nx, ny, nz, tx, ty, tz, sx, sy, sz;
   T dx, inv_dx, dt, s, pmass, vol0, k4;
   ModelParams<T> mp;
 };
 
-template <typename T>
+// MODEL >= 0 fixes the constitutive model at compile time (a batch with one
+// model); -1 reads it from the instance table.
+template <typename T, int MODEL = -1>
 __device__ __forceinline__ Inst<T> load_inst(const mpm_instance* inst, int i) {
   const mpm_instance& I = inst[i];
   Inst<T> r;
   r.p_off = I.p_off; r.g_off = I.g_off; r.c_off = I.c_off; r.t_off = I.t_off;
   r.n = I.n; r.nx = I.nx; r.ny = I.ny; r.nz = I.nz; r.tx = I.tx; r.ty = I.ty; r.tz = I.tz;
+  r.sx = I.shift[0]; r.sy = I.shift[1]; r.sz = I.shift[2];
   r.dx = (T)I.dx;
   r.inv_dx = (T)(1.0 / I.dx);
   r.dt = (T)I.dt;
@@ -76,7 +81,7 @@ __device__ __forceinline__ Inst<T> load_inst(const mpm_instance* inst, int i) {
   r.pmass = (T)I.pmass;
   r.vol0 = (T)I.vol0;
   r.k4 = (T)(4.0 / (I.dx * I.dx));
-  r.mp.kind = I.model;
+  r.mp.kind = MODEL >= 0 ? MODEL : I.model;
   for (int k = 0; k < 6; ++k) r.mp.p[k] = (T)I.prm[k];
   return r;
 }
@@ -100,6 +105,41 @@ __device__ __forceinline__ void st9(T* base, int64_t stride, int64_t p, int row,
   for (int k = 0; k < 9; ++k) base[(int64_t)(row + k) * stride + p] = m.a[k];
 }
 
+// 4-wide records (grid nodes, payload quads): one vector load per record.
+template <typename T> struct V4 { T x, y, z, w; };
+__device__ __forceinline__ V4<float> ld4(const float* p) {
+  const float4 v = *reinterpret_cast<const float4*>(p);
+  return {v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ V4<double> ld4(const double* p) {
+  const double2 a = *reinterpret_cast<const double2*>(p);
+  const double2 b = *reinterpret_cast<const double2*>(p + 2);
+  return {a.x, a.y, b.x, b.y};
+}
+__device__ __forceinline__ void st4(float* p, float x, float y, float z, float w) {
+  *reinterpret_cast<float4*>(p) = make_float4(x, y, z, w);
+}
+__device__ __forceinline__ void st4(double* p, double x, double y, double z, double w) {
+  *reinterpret_cast<double2*>(p) = make_double2(x, y);
+  *reinterpret_cast<double2*>(p + 2) = make_double2(z, w);
+}
+
+// Stencil nodes of a particle along each axis, clamped into the grid, with a
+// validity flag: out-of-grid nodes contribute nothing (transfer.py:51).
+struct AxisNodes {
+  int idx[3];
+  bool ok[3];
+};
+__device__ __forceinline__ AxisNodes axis_nodes(int base, int n) {
+  AxisNodes a;
+  for (int i = 0; i < 3; ++i) {
+    const int v = base + i;
+    a.ok[i] = v >= 0 && v < n;
+    a.idx[i] = v < 0 ? 0 : (v >= n ? n - 1 : v);
+  }
+  return a;
+}
+
 // Growth tensor of this step: Fg = I + a (G - I); a == 1 keeps G exactly.
 template <typename T>
 __device__ __forceinline__ M3<T> grown(const M3<T>& G, T a) {
@@ -115,6 +155,13 @@ __device__ __forceinline__ void stencil_base(const T x[3], T inv_dx, int base[3]
   for (int a = 0; a < 3; ++a) base[a] = cell_base(x[a], inv_dx, fx[a]);
 }
 
+// Instance of this block: blocks never straddle a 256-slot chunk (block sizes
+// divide 256), so the index is block-uniform and lives in a scalar register.
+__device__ __forceinline__ int block_inst(const int32_t* chunk_inst) {
+  const int64_t first = (int64_t)blockIdx.x * blockDim.x;
+  return __builtin_amdgcn_readfirstlane(chunk_inst[first / kChunk]);
+}
+
 // dyn row offsets
 constexpr int RX = 0, RV = 3, RC = 6, RF = 15;
 
@@ -126,29 +173,59 @@ __global__ void k_zero_i32(int32_t* a, int64_t n) {
   if (i < n) a[i] = 0;
 }
 
+// Bins are numbered tile-major: the (nx+2)(ny+2)(nz+2) bins of an instance
+// are grouped into P2G tiles of TX*TY*TZ bins; key = c_off + tile * TB + local
+// bin, so each tile's particles are contiguous in sorted order.  The last key
+// of an instance (c_off + tiles * TB) is the dead bin for particles with no
+// in-grid node.
+__device__ __forceinline__ int64_t bin_key(const mpm_instance& I, int bx, int by, int bz, int TX,
+                                           int TY, int TZ) {
+  bx += I.shift[0];
+  by += I.shift[1];
+  bz += I.shift[2];
+  const int itx = bx / TX, ity = by / TY, itz = bz / TZ;
+  const int64_t t = ((int64_t)itx * I.ty + ity) * I.tz + itz;
+  const int l = ((bx - itx * TX) * TY + (by - ity * TY)) * TZ + (bz - itz * TZ);
+  return I.c_off + t * (TX * TY * TZ) + l;
+}
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_sort_key(const mpm_instance* inst, const int32_t* slot_inst, int64_t n_slots,
-                           const T* dyn, int32_t* key, int32_t* slot, int32_t* count) {
+__global__ __launch_bounds__(256) void k_sort_key(const mpm_instance* inst, const int32_t* slot_inst,
+                                                  int64_t n_slots, const T* dyn, int TX, int TY,
+                                                  int TZ, int32_t* key, int32_t* slot,
+                                                  int32_t* count) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_slots) return;
-  const Inst<T> I = load_inst<T>(inst, slot_inst[p / kChunk]);
-  if (p - I.p_off >= I.n) {
-    key[p] = -1;
-    return;
+  const mpm_instance& I = inst[block_inst(slot_inst)];
+  int64_t k = -1;
+  if (p < n_slots && p - I.p_off < I.n) {
+    T x[3], fx[3];
+    int base[3];
+    ld3(dyn, n_slots, p, RX, x);
+    stencil_base(x, (T)(1.0 / I.dx), base, fx);
+    const int bx = base[0] + 2, by = base[1] + 2, bz = base[2] + 2;
+    if (bx >= 0 && bx < I.nx + 2 && by >= 0 && by < I.ny + 2 && bz >= 0 && bz < I.nz + 2)
+      k = bin_key(I, bx, by, bz, TX, TY, TZ);
+    else
+      k = I.c_off + (int64_t)I.tx * I.ty * I.tz * (TX * TY * TZ);  // dead bin
   }
-  T x[3], fx[3];
-  int base[3];
-  ld3(dyn, n_slots, p, RX, x);
-  stencil_base(x, I.inv_dx, base, fx);
-  const int bx = base[0] + 2, by = base[1] + 2, bz = base[2] + 2;
-  const int BX = I.nx + 2, BY = I.ny + 2, BZ = I.nz + 2;
-  int64_t k;
-  if (bx >= 0 && bx < BX && by >= 0 && by < BY && bz >= 0 && bz < BZ)
-    k = I.c_off + ((int64_t)bx * BY + by) * BZ + bz;
-  else
-    k = I.c_off + (int64_t)BX * BY * BZ;  // dead bin: no in-grid node
-  key[p] = (int32_t)k;
-  slot[p] = atomicAdd(&count[k], 1);
+  // One atomic per run of equal keys in adjacent lanes (particles arrive
+  // nearly sorted); the per-bin sort afterwards fixes the order in a bin.
+  const int lane = threadIdx.x & 63;
+  const int32_t k32 = (int32_t)k;
+  const int32_t prev = __shfl_up(k32, 1);
+  const bool head = (lane == 0) || (prev != k32);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+  const int my_head = 63 - __clzll(upto);
+  const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1));
+  const int next_head = after ? lane + __ffsll(after) : 64;
+  int32_t base_slot = 0;
+  if (head && k32 >= 0) base_slot = atomicAdd(&count[k32], next_head - lane);
+  base_slot = __shfl(base_slot, my_head);
+  if (p < n_slots) {
+    key[p] = k32;
+    if (k32 >= 0) slot[p] = base_slot + (lane - my_head);
+  }
 }
 
 // Exclusive scan of count[n] -> start[n + 1], in three passes.
@@ -230,20 +307,28 @@ __global__ void k_bin_sort(const int32_t* start, int32_t* sorted, int32_t* rank,
   for (int32_t j = s0; j < s1; ++j) rank[sorted[j]] = j;
 }
 
+template <typename T>
+__device__ __forceinline__ void store_payload(T* r, const T fx[3], const T q[3], const M3<T>& A) {
+  st4(r, fx[0], fx[1], fx[2], q[0]);
+  st4(r + 4, q[1], q[2], A.a[0], A.a[1]);
+  st4(r + 8, A.a[2], A.a[3], A.a[4], A.a[5]);
+  st4(r + 12, A.a[6], A.a[7], A.a[8], T(0));
+}
+
 // ---------------------------------------------------------------------------
 // forward prep: stress, grown volume, APIC payload (transfer.py:43-45,53)
 //   m_i += w pmass;  mom_i += w (P + A dpos),  P = pmass v,  A = pmass C + dt c tau,
 //   c = -4 inv_dx^2 vol0 det Fg,  dpos = (o - fx) dx  ->  A dpos = A' (o - fx),
 //   A' = A dx;  stored: fx, Q = P - A' fx, A'.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int MODEL>
 __global__ __launch_bounds__(256) void k_prep_fwd(const mpm_instance* inst, const int32_t* slot_inst, int64_t n_slots,
                            const T* dyn, const T* Fgp, const T* sched, const int32_t* rank,
                            T* payload) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_slots) return;
-  const int ii = slot_inst[p / kChunk];
-  const Inst<T> I = load_inst<T>(inst, ii);
+  const int ii = block_inst(slot_inst);
+  const Inst<T> I = load_inst<T, MODEL>(inst, ii);
   if (p - I.p_off >= I.n) return;
   T x[3], v[3], fx[3];
   int base[3];
@@ -257,41 +342,33 @@ __global__ __launch_bounds__(256) void k_prep_fwd(const mpm_instance* inst, cons
   const T c = -I.k4 * I.vol0 * det(Fg);
   M3<T> Ap;
   for (int k = 0; k < 9; ++k) Ap.a[k] = (I.pmass * C.a[k] + I.dt * (c * tau.a[k])) * I.dx;
-  const int64_t j = rank[p];
-  for (int a = 0; a < 3; ++a) {
-    const T q = I.pmass * v[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
-    payload[(int64_t)a * n_slots + j] = fx[a];
-    payload[(int64_t)(3 + a) * n_slots + j] = q;
-  }
-  for (int k = 0; k < 9; ++k) payload[(int64_t)(6 + k) * n_slots + j] = Ap.a[k];
+  T q[3];
+  for (int a = 0; a < 3; ++a)
+    q[a] = I.pmass * v[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
+  store_payload(payload + (int64_t)rank[p] * kRecord, fx, q, Ap);
 }
 
 // ---------------------------------------------------------------------------
 // P2G as a cell-binned segmented reduction into an LDS node tile.
-// Tile = TX*TY*TZ bins, one thread each; node tile (TX+2)(TY+2)(TZ+2).
-// partial: [n_tiles][4][TN] -- row 0 sum of weights (MASS), rows 1-3 sum of
-// w (Q + A' o).
+// Tile = TX*TY*TZ bins (contiguous keys), one thread each; node tile
+// (TX+2)(TY+2)(TZ+2).  partial: [n_tiles][4][TN] -- row 0 sum of weights
+// (MASS), rows 1-3 sum of w (Q + A' o).  Empty tiles return at once and are
+// skipped by the node kernels.
 // ---------------------------------------------------------------------------
 template <typename T, int TX, int TY, int TZ, bool MASS>
 __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* inst,
                                                         const int32_t* tile_inst,
                                                         const int32_t* bin_start,
-                                                        const T* payload, int64_t n_slots,
-                                                        T* partial) {
+                                                        const T* payload, T* partial) {
   constexpr int NY = TY + 2, NZ = TZ + 2, TN = (TX + 2) * NY * NZ;
   constexpr int NT = TX * TY * TZ;
   __shared__ T lds[4 * TN];
   const int64_t tile = blockIdx.x;
-  const int ii = tile_inst[tile];
-  const mpm_instance& I = inst[ii];
-  const int64_t lt = tile - I.t_off;
-  const int itz = (int)(lt % I.tz);
-  const int ity = (int)((lt / I.tz) % I.ty);
-  const int itx = (int)(lt / ((int64_t)I.tz * I.ty));
+  const mpm_instance& I = inst[__builtin_amdgcn_readfirstlane(tile_inst[tile])];
+  const int64_t key0 = I.c_off + (tile - I.t_off) * NT;
+  if (bin_start[key0] == bin_start[key0 + NT]) return;  // empty tile
   const int tid = threadIdx.x;
   const int lz = tid % TZ, ly = (tid / TZ) % TY, lx = tid / (TY * TZ);
-  const int BX = I.nx + 2, BY = I.ny + 2, BZ = I.nz + 2;
-  const int bx = itx * TX + lx, by = ity * TY + ly, bz = itz * TZ + lz;
 
   for (int i = tid; i < 4 * TN; i += NT) lds[i] = T(0);
 
@@ -301,37 +378,36 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
     aw[o] = T(0);
     am[o][0] = am[o][1] = am[o][2] = T(0);
   }
-  if (bx < BX && by < BY && bz < BZ) {
-    const int64_t key = I.c_off + ((int64_t)bx * BY + by) * BZ + bz;
-    const int32_t s0 = bin_start[key], s1 = bin_start[key + 1];
-    for (int32_t j = s0; j < s1; ++j) {
-      T f[kPayload];
+  const int32_t s0 = bin_start[key0 + tid], s1 = bin_start[key0 + tid + 1];
+  for (int32_t j = s0; j < s1; ++j) {
+    const T* r = payload + (int64_t)j * kRecord;
+    const V4<T> r0 = ld4(r), r1 = ld4(r + 4), r2 = ld4(r + 8), r3 = ld4(r + 12);
+    // fx = r0.xyz, Q = (r0.w, r1.x, r1.y), A' row-major = r1.zw r2.xyzw r3.xyz
+    const T q0 = r0.w, q1 = r1.x, q2 = r1.y;
+    const T A00 = r1.z, A01 = r1.w, A02 = r2.x, A10 = r2.y, A11 = r2.z, A12 = r2.w,
+            A20 = r3.x, A21 = r3.y, A22 = r3.z;
+    T w0[3], w1[3], w2[3];
+    bspline_w(r0.x, w0);
+    bspline_w(r0.y, w1);
+    bspline_w(r0.z, w2);
 #pragma unroll
-      for (int c = 0; c < kPayload; ++c) f[c] = payload[(int64_t)c * n_slots + j];
-      T w0[3], w1[3], w2[3];
-      bspline_w(f[0], w0);
-      bspline_w(f[1], w1);
-      bspline_w(f[2], w2);
-      // A' row-major at f[6..14]: A'(r, c) = f[6 + 3 r + c]
+    for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+      for (int jj = 0; jj < 3; ++jj) {
+        const T wij = w0[i] * w1[jj];
+        const T u0 = q0 + T(i) * A00 + T(jj) * A01;
+        const T u1 = q1 + T(i) * A10 + T(jj) * A11;
+        const T u2 = q2 + T(i) * A20 + T(jj) * A21;
 #pragma unroll
-        for (int jj = 0; jj < 3; ++jj) {
-          const T wij = w0[i] * w1[jj];
-          const T u0 = f[3] + T(i) * f[6] + T(jj) * f[7];
-          const T u1 = f[4] + T(i) * f[9] + T(jj) * f[10];
-          const T u2 = f[5] + T(i) * f[12] + T(jj) * f[13];
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            const int o = (i * 3 + jj) * 3 + k;
-            const T wt = wij * w2[k];
-            if (MASS) aw[o] += wt;
-            am[o][0] += wt * (u0 + T(k) * f[8]);
-            am[o][1] += wt * (u1 + T(k) * f[11]);
-            am[o][2] += wt * (u2 + T(k) * f[14]);
-          }
+        for (int k = 0; k < 3; ++k) {
+          const int o = (i * 3 + jj) * 3 + k;
+          const T wt = wij * w2[k];
+          if (MASS) aw[o] += wt;
+          am[o][0] += wt * (u0 + T(k) * A02);
+          am[o][1] += wt * (u1 + T(k) * A12);
+          am[o][2] += wt * (u2 + T(k) * A22);
         }
-    }
+      }
   }
 #pragma unroll
   for (int o = 0; o < 27; ++o) {
@@ -347,25 +423,29 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
   for (int i = tid; i < 4 * TN; i += NT) out[i] = lds[i];
 }
 
-// Sum of the tile partials covering node (i, j, k), tiles in ascending order.
+// Sum of the non-empty tile partials covering node (i, j, k), tiles in
+// ascending order.
 template <typename T, int TX, int TY, int TZ>
-__device__ __forceinline__ void gather_tiles(const Inst<T>& I, const T* partial, int i, int j, int k,
-                                             T acc[4]) {
-  constexpr int NY = TY + 2, NZ = TZ + 2, TN = (TX + 2) * NY * NZ;
+__device__ __forceinline__ void gather_tiles(const Inst<T>& I, const int32_t* bin_start,
+                                             const T* partial, int i, int j, int k, T acc[4]) {
+  constexpr int NY = TY + 2, NZ = TZ + 2, TN = (TX + 2) * NY * NZ, NT = TX * TY * TZ;
   acc[0] = acc[1] = acc[2] = acc[3] = T(0);
-  const int xhi = min((i + 2) / TX, I.tx - 1), yhi = min((j + 2) / TY, I.ty - 1),
-            zhi = min((k + 2) / TZ, I.tz - 1);
+  // tile ix covers nodes [ix TX - sx - 2, ix TX - sx + TX - 1]
+  const int xhi = min((i + 2 + I.sx) / TX, I.tx - 1), yhi = min((j + 2 + I.sy) / TY, I.ty - 1),
+            zhi = min((k + 2 + I.sz) / TZ, I.tz - 1);
   for (int ix = max(0, xhi - 2); ix <= xhi; ++ix) {
-    const int li = i - (ix * TX - 2);
+    const int li = i - (ix * TX - I.sx - 2);
     if (li < 0 || li >= TX + 2) continue;
     for (int iy = max(0, yhi - 2); iy <= yhi; ++iy) {
-      const int lj = j - (iy * TY - 2);
+      const int lj = j - (iy * TY - I.sy - 2);
       if (lj < 0 || lj >= TY + 2) continue;
       for (int iz = max(0, zhi - 2); iz <= zhi; ++iz) {
-        const int lk = k - (iz * TZ - 2);
+        const int lk = k - (iz * TZ - I.sz - 2);
         if (lk < 0 || lk >= TZ + 2) continue;
-        const int64_t tile = I.t_off + ((int64_t)ix * I.ty + iy) * I.tz + iz;
-        const T* src = partial + tile * (int64_t)(4 * TN) + (li * NY + lj) * NZ + lk;
+        const int64_t lt = ((int64_t)ix * I.ty + iy) * I.tz + iz;
+        const int64_t key0 = I.c_off + lt * NT;
+        if (bin_start[key0] == bin_start[key0 + NT]) continue;
+        const T* src = partial + (I.t_off + lt) * (int64_t)(4 * TN) + (li * NY + lj) * NZ + lk;
         acc[0] += src[0];
         acc[1] += src[TN];
         acc[2] += src[2 * TN];
@@ -376,67 +456,67 @@ __device__ __forceinline__ void gather_tiles(const Inst<T>& I, const T* partial,
 }
 
 // Grid update (transfer.py:56-59): v = (1 - damping) mom / m where m > 1e-12.
+// grid record per node: (m, vx, vy, vz).
 template <typename T, int TX, int TY, int TZ>
-__global__ void k_grid_fwd(const mpm_instance* inst, const int32_t* node_inst, int64_t n_nodes,
-                           const T* partial, T* grid) {
+__global__ __launch_bounds__(256) void k_grid_fwd(const mpm_instance* inst, const int32_t* node_inst,
+                                                  int64_t n_nodes, const int32_t* bin_start,
+                                                  const T* partial, T* grid) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_nodes) return;
-  const Inst<T> I = load_inst<T>(inst, node_inst[g / kChunk]);
+  const Inst<T> I = load_inst<T>(inst, block_inst(node_inst));
   const int64_t local = g - I.g_off;
   if (local >= (int64_t)I.nx * I.ny * I.nz) return;
   const int k = (int)(local % I.nz), j = (int)((local / I.nz) % I.ny),
             i = (int)(local / ((int64_t)I.nz * I.ny));
   T acc[4];
-  gather_tiles<T, TX, TY, TZ>(I, partial, i, j, k, acc);
+  gather_tiles<T, TX, TY, TZ>(I, bin_start, partial, i, j, k, acc);
   const T m = I.pmass * acc[0];
   T v[3] = {T(0), T(0), T(0)};
   if (m > T(kMassEps))
     for (int a = 0; a < 3; ++a) v[a] = I.s * acc[1 + a] / m;
-  grid[g] = m;
-  for (int a = 0; a < 3; ++a) grid[(int64_t)(1 + a) * n_nodes + g] = v[a];
+  st4(grid + 4 * g, m, v[0], v[1], v[2]);
 }
 
 // ---------------------------------------------------------------------------
 // G2P (transfer.py:61-77)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int32_t* slot_inst, int64_t n_slots,
-                      int64_t n_nodes, const T* dyn_in, const T* grid, T* dyn_out, int advect) {
+__global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int32_t* slot_inst,
+                                             int64_t n_slots, const T* dyn_in, const T* grid,
+                                             T* dyn_out, int advect) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_slots) return;
-  const Inst<T> I = load_inst<T>(inst, slot_inst[p / kChunk]);
+  const Inst<T> I = load_inst<T>(inst, block_inst(slot_inst));
   if (p - I.p_off >= I.n) return;
   T x[3], fx[3];
   int base[3];
   ld3(dyn_in, n_slots, p, RX, x);
-  const M3<T> F = ld9(dyn_in, n_slots, p, RF);
   stencil_base(x, I.inv_dx, base, fx);
+  const AxisNodes nx = axis_nodes(base[0], I.nx), ny = axis_nodes(base[1], I.ny),
+                  nz = axis_nodes(base[2], I.nz);
   T w[3][3];
   for (int a = 0; a < 3; ++a) bspline_w(fx[a], w[a]);
   T nv[3] = {T(0), T(0), T(0)};
   M3<T> nC = zero3<T>();
-  for (int i = 0; i < 3; ++i) {
-    const int ni = base[0] + i;
-    if (ni < 0 || ni >= I.nx) continue;
-    for (int j = 0; j < 3; ++j) {
-      const int nj = base[1] + j;
-      if (nj < 0 || nj >= I.ny) continue;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const int nk = base[2] + k;
-        if (nk < 0 || nk >= I.nz) continue;
-        const int64_t gi = I.g_off + ((int64_t)ni * I.ny + nj) * I.nz + nk;
+        const int64_t gi = I.g_off + ((int64_t)nx.idx[i] * I.ny + ny.idx[j]) * I.nz + nz.idx[k];
+        const V4<T> g = ld4(grid + 4 * gi);
+        const bool ok = nx.ok[i] && ny.ok[j] && nz.ok[k];
+        const T vi[3] = {ok ? g.y : T(0), ok ? g.z : T(0), ok ? g.w : T(0)};
         const T wt = w[0][i] * w[1][j] * w[2][k];
         const T d[3] = {(T(i) - fx[0]) * I.dx, (T(j) - fx[1]) * I.dx, (T(k) - fx[2]) * I.dx};
-        T vi[3];
-        for (int a = 0; a < 3; ++a) vi[a] = grid[(int64_t)(1 + a) * n_nodes + gi];
         const T sw = I.k4 * wt;
         for (int a = 0; a < 3; ++a) {
           nv[a] += wt * vi[a];
           for (int b = 0; b < 3; ++b) nC(a, b) += sw * (vi[a] * d[b]);
         }
       }
-    }
-  }
+  const M3<T> F = ld9(dyn_in, n_slots, p, RF);
   M3<T> A = scaled(nC, I.dt);
   A.a[0] += T(1); A.a[4] += T(1); A.a[8] += T(1);
   const M3<T> Fn = mul(A, F);
@@ -460,7 +540,7 @@ __global__ __launch_bounds__(256) void k_prep_adj(const mpm_instance* inst, cons
                            int advect) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_slots) return;
-  const Inst<T> I = load_inst<T>(inst, slot_inst[p / kChunk]);
+  const Inst<T> I = load_inst<T>(inst, block_inst(slot_inst));
   if (p - I.p_off >= I.n) return;
   T x[3], fx[3], nvb[3];
   int base[3];
@@ -478,173 +558,203 @@ __global__ __launch_bounds__(256) void k_prep_adj(const mpm_instance* inst, cons
   const M3<T> FF = mul_nt(Fnb, F);
   M3<T> Ap;
   for (int k = 0; k < 9; ++k) Ap.a[k] = I.k4 * (Cnb.a[k] + I.dt * FF.a[k]) * I.dx;
-  const int64_t j = rank[p];
-  for (int a = 0; a < 3; ++a) {
-    const T q = nvb[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
-    payload[(int64_t)a * n_slots + j] = fx[a];
-    payload[(int64_t)(3 + a) * n_slots + j] = q;
-  }
-  for (int k = 0; k < 9; ++k) payload[(int64_t)(6 + k) * n_slots + j] = Ap.a[k];
+  T q[3];
+  for (int a = 0; a < 3; ++a)
+    q[a] = nvb[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
+  store_payload(payload + (int64_t)rank[p] * kRecord, fx, q, Ap);
 }
 
 // reverse step, part 2 (transfer.py:121-127): grid adjoint.
-//   gadj row 0 = gm_bar, rows 1-3 = gmom_bar.
+//   gadj record per node: (gm_bar, gmom_bar x, y, z).
 template <typename T, int TX, int TY, int TZ>
-__global__ void k_grid_adj(const mpm_instance* inst, const int32_t* node_inst, int64_t n_nodes,
-                           const T* partial, const T* grid, T* gadj, int advect) {
+__global__ __launch_bounds__(256) void k_grid_adj(const mpm_instance* inst, const int32_t* node_inst,
+                                                  int64_t n_nodes, const int32_t* bin_start,
+                                                  const T* partial, const T* grid, T* gadj,
+                                                  int advect) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_nodes) return;
-  const Inst<T> I = load_inst<T>(inst, node_inst[g / kChunk]);
+  const Inst<T> I = load_inst<T>(inst, block_inst(node_inst));
   const int64_t local = g - I.g_off;
   if (local >= (int64_t)I.nx * I.ny * I.nz) return;
   const int k = (int)(local % I.nz), j = (int)((local / I.nz) % I.ny),
             i = (int)(local / ((int64_t)I.nz * I.ny));
   T acc[4];
-  gather_tiles<T, TX, TY, TZ>(I, partial, i, j, k, acc);
-  const T m = grid[g];
+  gather_tiles<T, TX, TY, TZ>(I, bin_start, partial, i, j, k, acc);
+  const V4<T> gr = ld4(grid + 4 * g);
+  const T m = gr.x;
   T mb[3] = {T(0), T(0), T(0)};
   T gmb = T(0);
   if (m > T(kMassEps)) {
     for (int a = 0; a < 3; ++a) mb[a] = I.s * acc[1 + a] / m;
-    if (advect) {
-      T dot = T(0);
-      for (int a = 0; a < 3; ++a) dot += acc[1 + a] * grid[(int64_t)(1 + a) * n_nodes + g];
-      gmb = -dot / m;
-    }
+    if (advect) gmb = -(acc[1] * gr.y + acc[2] * gr.z + acc[3] * gr.w) / m;
   }
-  gadj[g] = gmb;
-  for (int a = 0; a < 3; ++a) gadj[(int64_t)(1 + a) * n_nodes + g] = mb[a];
+  st4(gadj + 4 * g, gmb, mb[0], mb[1], mb[2]);
 }
 
 // reverse step, part 3 (transfer.py:100-119 x_bar terms, :129-156):
 // P2G-adjoint gather, advection terms, constitutive VJP, Fg and theta paths.
-template <typename T>
+//
+// With dpos = (o - fx) dx for stencil offset o, the per-node factors are
+// affine in o and are built incrementally:
+//   nv_b + 4 inv_dx^2 C_b' dpos = q1 + P1 o,   P1 = 4 inv_dx^2 dx C_b',  q1 = nv_b - P1 fx
+//   pmass v + A dpos            = q2 + A' o,   A' = A dx,                  q2 = pmass v - A' fx
+// so x_bar's grad-w coefficient is v_i.(q1 + P1 o) + mb_i.(q2 + A' o) + gm_b,i pmass.
+// F, Fg and C are reloaded after the node loop rather than kept live in it.
+template <typename T, int MODEL>
 __global__ __launch_bounds__(256) void k_p2g_adj(const mpm_instance* inst, const int32_t* slot_inst,
-                                                 int64_t n_slots, int64_t n_nodes, const T* dyn_in,
-                                                 const T* Fgp, const T* sched, const T* dyn_out,
-                                                 const T* grid, const T* gadj, const T* bar_out,
-                                                 T* bar_in, T* Fg_bar, double* theta_acc,
-                                                 int advect) {
+                                                 int64_t n_slots, const T* dyn_in, const T* Fgp,
+                                                 const T* sched, const T* dyn_out, const T* grid,
+                                                 const T* gadj, const T* bar_out, T* bar_in,
+                                                 T* Fg_bar, double* theta_acc, int advect) {
   __shared__ double red[2][256];
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t pc = (p < n_slots) ? p : n_slots - 1;
-  const int ii = slot_inst[pc / kChunk];
-  const Inst<T> I = load_inst<T>(inst, ii);
+  const int ii = block_inst(slot_inst);
+  const Inst<T> I = load_inst<T, MODEL>(inst, ii);
   const bool live = (p < n_slots) && (p - I.p_off < I.n);
   T th[2] = {T(0), T(0)};
   if (live) {
-    T x[3], v[3], fx[3];
+    T fx[3];
     int base[3];
-    ld3(dyn_in, n_slots, p, RX, x);
-    ld3(dyn_in, n_slots, p, RV, v);
-    const M3<T> C = ld9(dyn_in, n_slots, p, RC);
-    const M3<T> F = ld9(dyn_in, n_slots, p, RF);
+    {
+      T x[3];
+      ld3(dyn_in, n_slots, p, RX, x);
+      stencil_base(x, I.inv_dx, base, fx);
+    }
     const T a_t = sched ? sched[ii] : T(1);
-    const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), a_t);
-    const M3<T> Cn = ld9(dyn_out, n_slots, p, RC);
-    stencil_base(x, I.inv_dx, base, fx);
+    T c, tau6[6];                 // tau symmetric: 00 01 02 11 12 22
+    M3<T> Ap, P1;
+    T q1[3], q2[3];
+    {
+      const M3<T> C = ld9(dyn_in, n_slots, p, RC);
+      const M3<T> F = ld9(dyn_in, n_slots, p, RF);
+      const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), a_t);
+      const M3<T> tau = kirchhoff(I.mp, F, Fg, C);
+      c = -I.k4 * I.vol0 * det(Fg);
+      tau6[0] = tau.a[0]; tau6[1] = tau.a[1]; tau6[2] = tau.a[2];
+      tau6[3] = tau.a[4]; tau6[4] = tau.a[5]; tau6[5] = tau.a[8];
+      for (int k = 0; k < 9; ++k) Ap.a[k] = (I.pmass * C.a[k] + I.dt * (c * tau.a[k])) * I.dx;
+      T v[3];
+      ld3(dyn_in, n_slots, p, RV, v);
+      for (int a = 0; a < 3; ++a)
+        q2[a] = I.pmass * v[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
+      const M3<T> Fnb = ld9(bar_out, n_slots, p, RF);
+      const M3<T> Cnb = ld9(bar_out, n_slots, p, RC);
+      const M3<T> FF = mul_nt(Fnb, F);
+      const T s1 = I.k4 * I.dx;
+      for (int k = 0; k < 9; ++k) P1.a[k] = s1 * (Cnb.a[k] + I.dt * FF.a[k]);
+      T nvb[3];
+      ld3(bar_out, n_slots, p, RV, nvb);
+      if (advect) {
+        T xnb[3];
+        ld3(bar_out, n_slots, p, RX, xnb);
+        for (int a = 0; a < 3; ++a) nvb[a] += I.dt * xnb[a];
+      }
+      for (int a = 0; a < 3; ++a)
+        q1[a] = nvb[a] - (P1(a, 0) * fx[0] + P1(a, 1) * fx[1] + P1(a, 2) * fx[2]);
+    }
+    const AxisNodes nx = axis_nodes(base[0], I.nx), ny = axis_nodes(base[1], I.ny),
+                    nz = axis_nodes(base[2], I.nz);
     T w[3][3], dw[3][3];
     for (int a = 0; a < 3; ++a) {
       bspline_w(fx[a], w[a]);
       bspline_dw(fx[a], dw[a]);
     }
-    const M3<T> tau = kirchhoff(I.mp, F, Fg, C);
-    const T detG = det(Fg);
-    const T c = -I.k4 * I.vol0 * detG;
-    const M3<T> sk = scaled(tau, c);
-    // A = pmass C + dt sk (momentum affine map of the forward P2G)
-    M3<T> A;
-    for (int k = 0; k < 9; ++k) A.a[k] = I.pmass * C.a[k] + I.dt * sk.a[k];
-
-    // cotangents of the step outputs
-    T xnb[3], nvb[3];
-    ld3(bar_out, n_slots, p, RX, xnb);
-    ld3(bar_out, n_slots, p, RV, nvb);
-    if (advect)
-      for (int a = 0; a < 3; ++a) nvb[a] += I.dt * xnb[a];
-    const M3<T> Cnb = ld9(bar_out, n_slots, p, RC);
-    const M3<T> Fnb = ld9(bar_out, n_slots, p, RF);
-    M3<T> Cb = mul_nt(Fnb, F);
-    for (int k = 0; k < 9; ++k) Cb.a[k] = Cnb.a[k] + I.dt * Cb.a[k];
-    M3<T> IC = scaled(Cn, I.dt);
-    IC.a[0] += T(1); IC.a[4] += T(1); IC.a[8] += T(1);
-    M3<T> Fb = mul_tn(IC, Fnb);  // (I + dt Cn)^T Fn_b
-
     T xb[3] = {T(0), T(0), T(0)};
-    if (advect)
-      for (int a = 0; a < 3; ++a) xb[a] = xnb[a];
     T vb[3] = {T(0), T(0), T(0)};
     M3<T> MD = zero3<T>();
     T Sv[3] = {T(0), T(0), T(0)};  // sum w v_i
-    T Sm[3] = {T(0), T(0), T(0)};  // sum w mb_i
-
-    for (int i = 0; i < 3; ++i) {
-      const int ni = base[0] + i;
-      if (ni < 0 || ni >= I.nx) continue;
-      for (int j = 0; j < 3; ++j) {
-        const int nj = base[1] + j;
-        if (nj < 0 || nj >= I.ny) continue;
-        for (int k = 0; k < 3; ++k) {
-          const int nk = base[2] + k;
-          if (nk < 0 || nk >= I.nz) continue;
-          const int64_t gi = I.g_off + ((int64_t)ni * I.ny + nj) * I.nz + nk;
-          const T wt = w[0][i] * w[1][j] * w[2][k];
-          const T d[3] = {(T(i) - fx[0]) * I.dx, (T(j) - fx[1]) * I.dx, (T(k) - fx[2]) * I.dx};
-          T mb[3];
-          for (int a = 0; a < 3; ++a) mb[a] = gadj[(int64_t)(1 + a) * n_nodes + gi];
+    // The (i, j) loops stay rolled (9 rows of 3 nodes) to bound the gathered
+    // values in flight; per-axis values for row (i, j) are picked by select,
+    // not by dynamic register indexing.
+#pragma unroll 1
+    for (int ij = 0; ij < 9; ++ij) {
+      const int i = ij / 3, j = ij - 3 * (ij / 3);
+      const T wxi = i == 0 ? w[0][0] : (i == 1 ? w[0][1] : w[0][2]);
+      const T dxi = i == 0 ? dw[0][0] : (i == 1 ? dw[0][1] : dw[0][2]);
+      const T wyj = j == 0 ? w[1][0] : (j == 1 ? w[1][1] : w[1][2]);
+      const T dyj = j == 0 ? dw[1][0] : (j == 1 ? dw[1][1] : dw[1][2]);
+      const int nxi = i == 0 ? nx.idx[0] : (i == 1 ? nx.idx[1] : nx.idx[2]);
+      const int nyj = j == 0 ? ny.idx[0] : (j == 1 ? ny.idx[1] : ny.idx[2]);
+      const bool okxy = (i == 0 ? nx.ok[0] : (i == 1 ? nx.ok[1] : nx.ok[2])) &&
+                        (j == 0 ? ny.ok[0] : (j == 1 ? ny.ok[1] : ny.ok[2]));
+      const T Ti = T(i), Tj = T(j);
+      const T wij = wxi * wyj;
+      const T d0 = (Ti - fx[0]) * I.dx, d1 = (Tj - fx[1]) * I.dx;
+      T u1[3], u2[3];
+      for (int a = 0; a < 3; ++a) {
+        u1[a] = q1[a] + Ti * P1(a, 0) + Tj * P1(a, 1);
+        u2[a] = q2[a] + Ti * Ap(a, 0) + Tj * Ap(a, 1);
+      }
+      const int64_t row = I.g_off + ((int64_t)nxi * I.ny + nyj) * I.nz;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int64_t gi = row + nz.idx[k];
+        const bool ok = okxy && nz.ok[k];
+        const V4<T> ga = ld4(gadj + 4 * gi);
+        const T wt = wij * w[2][k];
+        const T d[3] = {d0, d1, (T(k) - fx[2]) * I.dx};
+        const T mb[3] = {ok ? ga.y : T(0), ok ? ga.z : T(0), ok ? ga.w : T(0)};
+        for (int a = 0; a < 3; ++a) {
+          vb[a] += wt * mb[a];
+          for (int b = 0; b < 3; ++b) MD(a, b) += wt * (mb[a] * d[b]);
+        }
+        if (advect) {
+          const V4<T> gv = ld4(grid + 4 * gi);
+          const T vi[3] = {ok ? gv.y : T(0), ok ? gv.z : T(0), ok ? gv.w : T(0)};
+          const T gmb = ok ? ga.x : T(0);
+          const T gw[3] = {dxi * wyj * w[2][k] * I.inv_dx, wxi * dyj * w[2][k] * I.inv_dx,
+                           wij * dw[2][k] * I.inv_dx};
+          T alpha = gmb * I.pmass;
+          for (int a = 0; a < 3; ++a)
+            alpha += vi[a] * (u1[a] + T(k) * P1(a, 2)) + mb[a] * (u2[a] + T(k) * Ap(a, 2));
           for (int a = 0; a < 3; ++a) {
-            vb[a] += wt * mb[a];
-            for (int b = 0; b < 3; ++b) MD(a, b) += wt * (mb[a] * d[b]);
-          }
-          if (advect) {
-            T vi[3];
-            for (int a = 0; a < 3; ++a) vi[a] = grid[(int64_t)(1 + a) * n_nodes + gi];
-            const T gmb = gadj[gi];
-            const T gw[3] = {dw[0][i] * w[1][j] * w[2][k] * I.inv_dx,
-                             w[0][i] * dw[1][j] * w[2][k] * I.inv_dx,
-                             w[0][i] * w[1][j] * dw[2][k] * I.inv_dx};
-            // G2P adjoint:  (nv_b . v_i) + 4 inv_dx^2 <C_b', v_i dpos^T>
-            // P2G adjoint:  mb . (pmass v + A dpos) + gm_b pmass
-            T alpha = gmb * I.pmass;
-            for (int a = 0; a < 3; ++a) {
-              const T Cd = Cb(a, 0) * d[0] + Cb(a, 1) * d[1] + Cb(a, 2) * d[2];
-              const T Ad = A(a, 0) * d[0] + A(a, 1) * d[1] + A(a, 2) * d[2];
-              alpha += nvb[a] * vi[a] + I.k4 * vi[a] * Cd + mb[a] * (I.pmass * v[a] + Ad);
-            }
-            for (int a = 0; a < 3; ++a) {
-              xb[a] += alpha * gw[a];
-              Sv[a] += wt * vi[a];
-              Sm[a] += wt * mb[a];
-            }
+            xb[a] += alpha * gw[a];
+            Sv[a] += wt * vi[a];
           }
         }
       }
     }
     if (advect) {
-      // - 4 inv_dx^2 C_b'^T (sum w v_i) - A^T (sum w mb_i)
+      // xn_b - (4 inv_dx^2 C_b')^T (sum w v_i) - A^T (sum w mb_i), A = A'/dx
+      T xnb[3];
+      ld3(bar_out, n_slots, p, RX, xnb);
       for (int a = 0; a < 3; ++a)
-        xb[a] -= I.k4 * (Cb(0, a) * Sv[0] + Cb(1, a) * Sv[1] + Cb(2, a) * Sv[2]) +
-                 (A(0, a) * Sm[0] + A(1, a) * Sm[1] + A(2, a) * Sm[2]);
+        xb[a] += xnb[a] - ((P1(0, a) * Sv[0] + P1(1, a) * Sv[1] + P1(2, a) * Sv[2]) +
+                           (Ap(0, a) * vb[0] + Ap(1, a) * vb[1] + Ap(2, a) * vb[2])) * I.inv_dx;
     }
     for (int a = 0; a < 3; ++a) vb[a] *= I.pmass;
-    M3<T> Cbo = scaled(MD, I.pmass);
-    const M3<T> skb = scaled(MD, I.dt);
-    const M3<T> taub = scaled(skb, c);
-    M3<T> Fpb, Fgb, Cbar;
-    kirchhoff_vjp(I.mp, F, Fg, C, taub, Fpb, Fgb, Cbar, th);
-    for (int k = 0; k < 9; ++k) {
-      Fb.a[k] += Fpb.a[k];
-      Cbo.a[k] += Cbar.a[k];
-    }
-    // grown-volume path: <sk_b, tau> (-k4 vol0) det Fg Fg^{-T}
-    const M3<T> GiT = inv_t(Fg);
-    const T vol = frob(skb, tau) * (-I.k4 * I.vol0) * detG;
-    M3<T> Gb = ld9(Fg_bar, n_slots, p, 0);
-    for (int k = 0; k < 9; ++k) Gb.a[k] += a_t * (Fgb.a[k] + vol * GiT.a[k]);
     st3(bar_in, n_slots, p, RX, xb);
     st3(bar_in, n_slots, p, RV, vb);
-    st9(bar_in, n_slots, p, RC, Cbo);
-    st9(bar_in, n_slots, p, RF, Fb);
+    // sk_b = dt MD, tau_b = c sk_b; grown-volume path <sk_b, tau>(-k4 vol0) det Fg Fg^{-T}
+    const M3<T> skb = scaled(MD, I.dt);
+    const T skt = skb.a[0] * tau6[0] + (skb.a[1] + skb.a[3]) * tau6[1] +
+                  (skb.a[2] + skb.a[6]) * tau6[2] + skb.a[4] * tau6[3] +
+                  (skb.a[5] + skb.a[7]) * tau6[4] + skb.a[8] * tau6[5];
+    const M3<T> C = ld9(dyn_in, n_slots, p, RC);
+    const M3<T> F = ld9(dyn_in, n_slots, p, RF);
+    const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), a_t);
+    M3<T> Fpb, Fgb, Cbar;
+    kirchhoff_vjp(I.mp, F, Fg, C, scaled(skb, c), Fpb, Fgb, Cbar, th);
+    {
+      M3<T> Cbo = scaled(MD, I.pmass);
+      for (int k = 0; k < 9; ++k) Cbo.a[k] += Cbar.a[k];
+      st9(bar_in, n_slots, p, RC, Cbo);
+    }
+    {
+      // F_b = (I + dt Cn)^T Fn_b + stress path
+      const M3<T> Cn = ld9(dyn_out, n_slots, p, RC);
+      const M3<T> Fnb = ld9(bar_out, n_slots, p, RF);
+      M3<T> IC = scaled(Cn, I.dt);
+      IC.a[0] += T(1); IC.a[4] += T(1); IC.a[8] += T(1);
+      const M3<T> Fb = mul_tn(IC, Fnb);
+      for (int k = 0; k < 9; ++k) Fpb.a[k] += Fb.a[k];
+      st9(bar_in, n_slots, p, RF, Fpb);
+    }
+    const T detG = det(Fg);
+    const M3<T> GiT = inv_t(Fg);
+    const T vol = skt * (-I.k4 * I.vol0) * detG;
+    M3<T> Gb = ld9(Fg_bar, n_slots, p, 0);
+    for (int k = 0; k < 9; ++k) Gb.a[k] += a_t * (Fgb.a[k] + vol * GiT.a[k]);
     st9(Fg_bar, n_slots, p, 0, Gb);
   }
   if (theta_acc != nullptr) {  // fixed-order block reduction, one slot per 256 particles
@@ -658,7 +768,7 @@ __global__ __launch_bounds__(256) void k_p2g_adj(const mpm_instance* inst, const
       }
       __syncthreads();
     }
-    if (threadIdx.x == 0 && p < n_slots) {
+    if (threadIdx.x == 0) {
       theta_acc[2 * blockIdx.x] += red[0][0];
       theta_acc[2 * blockIdx.x + 1] += red[1][0];
     }
@@ -668,16 +778,17 @@ __global__ __launch_bounds__(256) void k_p2g_adj(const mpm_instance* inst, const
 // ---------------------------------------------------------------------------
 // constitutive seam kernels
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int MODEL>
 __global__ __launch_bounds__(256) void k_stress(ModelParams<T> mp, int64_t n, int64_t stride, const T* F, const T* Fg,
                          const T* C, T* tau) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const M3<T> Cm = C ? ld9(C, stride, p, 0) : zero3<T>();
+  mp.kind = MODEL;
   st9(tau, stride, p, 0, kirchhoff(mp, ld9(F, stride, p, 0), ld9(Fg, stride, p, 0), Cm));
 }
 
-template <typename T>
+template <typename T, int MODEL>
 __global__ __launch_bounds__(256) void k_stress_vjp(ModelParams<T> mp, int64_t n, int64_t stride, const T* F,
                              const T* Fg, const T* C, const T* tb, T* Fb, T* Fgb, T* Cb, T* thb) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -685,6 +796,7 @@ __global__ __launch_bounds__(256) void k_stress_vjp(ModelParams<T> mp, int64_t n
   const M3<T> Cm = C ? ld9(C, stride, p, 0) : zero3<T>();
   M3<T> fb, gb, cb;
   T th[2];
+  mp.kind = MODEL;
   kirchhoff_vjp(mp, ld9(F, stride, p, 0), ld9(Fg, stride, p, 0), Cm, ld9(tb, stride, p, 0), fb, gb,
                 cb, th);
   st9(Fb, stride, p, 0, fb);
@@ -738,7 +850,7 @@ int64_t carve(const mpm_batch* b, size_t tsz, char* base, Workspace* w) {
   ws.slot = (int32_t*)take(4 * b->n_slots);
   ws.count = (int32_t*)take(4 * b->n_bins);
   ws.bsum = (int32_t*)take(4 * (scan_blocks(b->n_bins) + 1));
-  ws.payload = take((int64_t)tsz * kPayload * b->n_slots);
+  ws.payload = take((int64_t)tsz * kRecord * b->n_slots);
   ws.partial = take((int64_t)tsz * 4 * tile_nodes(b) * b->n_tiles);
   ws.gadj = take((int64_t)tsz * 4 * b->n_nodes);
   if (w) *w = ws;
@@ -762,6 +874,18 @@ int validate(const mpm_batch* b) {
 }
 
 inline dim3 grid1(int64_t n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+// Calls f with std::integral_constant<int, M>: the single model of the batch,
+// or -1 (read per instance) when the batch mixes models.
+template <typename F>
+void with_model(int32_t mask, F&& f) {
+  switch (mask) {
+    case 1 << MPM_NEO_HOOKEAN: f(std::integral_constant<int, MPM_NEO_HOOKEAN>()); break;
+    case 1 << MPM_HENCKY: f(std::integral_constant<int, MPM_HENCKY>()); break;
+    case 1 << MPM_HERSCHEL_BULKLEY: f(std::integral_constant<int, MPM_HERSCHEL_BULKLEY>()); break;
+    default: f(std::integral_constant<int, -1>()); break;
+  }
+}
 
 // Kernel timer (mpm_kernel_timer): events around the armed kernel's launches.
 struct KernelTimer {
@@ -807,7 +931,8 @@ int sort_impl(const mpm_batch* b, const T* dyn, int32_t* sortbuf, const Workspac
   TimedLaunch tl(MPM_K_SORT, st);
   k_zero_i32<<<grid1(b->n_bins, 256), 256, 0, st>>>(w.count, b->n_bins);
   k_sort_key<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn,
-                                                       w.key, w.slot, w.count);
+                                                       b->tile[0], b->tile[1], b->tile[2], w.key,
+                                                       w.slot, w.count);
   const int64_t nb = scan_blocks(b->n_bins);
   k_scan_blocks<<<dim3((unsigned)nb), kScanBlock, 0, st>>>(w.count, start, w.bsum, b->n_bins);
   k_scan_sums<<<1, 1024, 0, st>>>(w.bsum, nb, start, b->n_bins);
@@ -823,11 +948,11 @@ void launch_tile_fwd(const mpm_batch* b, const int32_t* start, const Workspace& 
   {
   TimedLaunch tl(MPM_K_P2G_FWD, st);
   k_p2g_tile<T, TX, TY, TZ, true><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
-      b->inst, b->tile_inst, start, (const T*)w.payload, b->n_slots, (T*)w.partial);
+      b->inst, b->tile_inst, start, (const T*)w.payload, (T*)w.partial);
   }
   TimedLaunch tl(MPM_K_GRID_FWD, st);
   k_grid_fwd<T, TX, TY, TZ><<<grid1(b->n_nodes, 256), 256, 0, st>>>(
-      b->inst, b->node_inst, b->n_nodes, (const T*)w.partial, grid);
+      b->inst, b->node_inst, b->n_nodes, start, (const T*)w.partial, grid);
 }
 
 template <typename T, int TX, int TY, int TZ>
@@ -836,11 +961,11 @@ void launch_tile_adj(const mpm_batch* b, const int32_t* start, const Workspace& 
   {
   TimedLaunch tl(MPM_K_P2G_ADJ_TILE, st);
   k_p2g_tile<T, TX, TY, TZ, false><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
-      b->inst, b->tile_inst, start, (const T*)w.payload, b->n_slots, (T*)w.partial);
+      b->inst, b->tile_inst, start, (const T*)w.payload, (T*)w.partial);
   }
   TimedLaunch tl(MPM_K_GRID_ADJ, st);
   k_grid_adj<T, TX, TY, TZ><<<grid1(b->n_nodes, 256), 256, 0, st>>>(
-      b->inst, b->node_inst, b->n_nodes, (const T*)w.partial, grid, (T*)w.gadj, advect);
+      b->inst, b->node_inst, b->n_nodes, start, (const T*)w.partial, grid, (T*)w.gadj, advect);
 }
 
 template <typename T>
@@ -883,13 +1008,15 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   const int bs = b->block;
   {
   TimedLaunch tl(MPM_K_PREP_FWD, st);
-  k_prep_fwd<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in,
-                                                      Fg, sched, rank, (T*)w.payload);
+  with_model(b->model_mask, [&](auto M) {
+    k_prep_fwd<T, decltype(M)::value><<<grid1(b->n_slots, bs), bs, 0, st>>>(
+        b->inst, b->slot_inst, b->n_slots, dyn_in, Fg, sched, rank, (T*)w.payload);
+  });
   }
   dispatch_fwd<T>(tile_index(b), b, start, w, grid, st);
   TimedLaunch tl(MPM_K_G2P, st);
-  k_g2p<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, b->n_nodes,
-                                                 dyn_in, grid, dyn_out, advect);
+  k_g2p<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in, grid,
+                                                 dyn_out, advect);
   return check_launch("step_fwd");
 }
 
@@ -915,9 +1042,11 @@ int step_bwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   }
   dispatch_adj<T>(tile_index(b), b, start, w, grid, advect, st);
   TimedLaunch tl(MPM_K_P2G_ADJ, st);
-  k_p2g_adj<T><<<grid1(b->n_slots, 256), 256, 0, st>>>(
-      b->inst, b->slot_inst, b->n_slots, b->n_nodes, dyn_in, Fg, sched, dyn_out, grid,
-      (const T*)w.gadj, bar_out, bar_in, Fg_bar, theta_acc, advect);
+  with_model(b->model_mask, [&](auto M) {
+    k_p2g_adj<T, decltype(M)::value><<<grid1(b->n_slots, 256), 256, 0, st>>>(
+        b->inst, b->slot_inst, b->n_slots, dyn_in, Fg, sched, dyn_out, grid, (const T*)w.gadj,
+        bar_out, bar_in, Fg_bar, theta_acc, advect);
+  });
   return check_launch("step_bwd");
 }
 
@@ -932,7 +1061,10 @@ int stress_impl(int32_t model, const double* prm6, int64_t n, int64_t stride, co
   ModelParams<T> mp;
   mp.kind = model;
   for (int k = 0; k < 6; ++k) mp.p[k] = (T)prm6[k];
-  k_stress<T><<<grid1(n, 256), 256, 0, (hipStream_t)stream>>>(mp, n, stride, F, Fg, C, tau);
+  with_model(1 << model, [&](auto M) {
+    k_stress<T, decltype(M)::value><<<grid1(n, 256), 256, 0, (hipStream_t)stream>>>(
+        mp, n, stride, F, Fg, C, tau);
+  });
   return check_launch("stress");
 }
 
@@ -948,8 +1080,10 @@ int stress_vjp_impl(int32_t model, const double* prm6, int64_t n, int64_t stride
   ModelParams<T> mp;
   mp.kind = model;
   for (int k = 0; k < 6; ++k) mp.p[k] = (T)prm6[k];
-  k_stress_vjp<T><<<grid1(n, 256), 256, 0, (hipStream_t)stream>>>(mp, n, stride, F, Fg, C, tb, Fb,
-                                                                   Fgb, Cb, thb);
+  with_model(1 << model, [&](auto M) {
+    k_stress_vjp<T, decltype(M)::value><<<grid1(n, 256), 256, 0, (hipStream_t)stream>>>(
+        mp, n, stride, F, Fg, C, tb, Fb, Fgb, Cb, thb);
+  });
   return check_launch("stress_vjp");
 }
 

@@ -52,12 +52,22 @@ class InstanceSpec:
     vol0: float
     kind: int
     prm: tuple
+    bin_lo: tuple = None   # lowest occupied bin per axis at the start (tile alignment)
 
     @staticmethod
-    def from_cfg(n, cfg, model):
+    def from_cfg(n, cfg, model, pts=None):
+        """``pts`` (optional rest positions) aligns the P2G tiles with the lowest
+        occupied bin of each axis."""
         kind, prm = model_spec(model)
+        lo = None
+        if pts is not None:
+            x = np.asarray(pts.detach().cpu() if isinstance(pts, torch.Tensor) else pts,
+                           dtype=np.float64)
+            if len(x):
+                lo = tuple(int(v) for v in
+                           np.maximum(np.floor(x * (1.0 / cfg.dx) - 0.5).min(0) + 2, 0))
         return InstanceSpec(int(n), tuple(cfg.shape), float(cfg.dx), float(cfg.dt),
-                            float(cfg.damping), float(cfg.pmass), float(cfg.vol0), kind, prm)
+                            float(cfg.damping), float(cfg.pmass), float(cfg.vol0), kind, prm, lo)
 
 
 def _ceil(a, b):
@@ -79,9 +89,11 @@ class BatchPlan:
         self.p_off, self.g_off = [], []
         for i, s in enumerate(self.specs):
             nx, ny, nz = s.shape
-            tx, ty, tz = (_ceil(nx + 2, self.tile[0]), _ceil(ny + 2, self.tile[1]),
-                          _ceil(nz + 2, self.tile[2]))
+            sh = [0, 0, 0] if s.bin_lo is None else [(-int(b)) % t for b, t in zip(s.bin_lo, self.tile)]
+            tx, ty, tz = (_ceil(nx + 2 + sh[0], self.tile[0]), _ceil(ny + 2 + sh[1], self.tile[1]),
+                          _ceil(nz + 2 + sh[2], self.tile[2]))
             I = insts[i]
+            I.shift[0], I.shift[1], I.shift[2] = sh
             I.p_off, I.g_off, I.c_off, I.t_off = p_off, g_off, c_off, t_off
             I.n, I.nx, I.ny, I.nz, I.tx, I.ty, I.tz = s.n, nx, ny, nz, tx, ty, tz
             I.model = s.kind
@@ -97,7 +109,7 @@ class BatchPlan:
             tile_inst += [i] * (tx * ty * tz)
             p_off += ps
             g_off += gs
-            c_off += (nx + 2) * (ny + 2) * (nz + 2) + 1
+            c_off += tx * ty * tz * int(np.prod(self.tile)) + 1
             t_off += tx * ty * tz
         self.n_slots, self.n_nodes, self.n_bins, self.n_tiles = p_off, g_off, c_off, t_off
         self.n_inst = len(self.specs)
@@ -113,6 +125,7 @@ class BatchPlan:
         c.n_slots, c.n_nodes, c.n_bins, c.n_tiles = (self.n_slots, self.n_nodes, self.n_bins,
                                                      self.n_tiles)
         c.block = self.block
+        c.model_mask = int(np.bitwise_or.reduce([1 << s.kind for s in self.specs]))
         c.inst = self.d_inst.data_ptr()
         c.slot_inst = self.d_slot_inst.data_ptr()
         c.node_inst = self.d_node_inst.data_ptr()
@@ -143,43 +156,82 @@ class BatchPlan:
         return torch.zeros(9, self.n_slots, dtype=dtype, device=self.device)
 
     def new_grid(self, dtype):
-        return torch.zeros(4, self.n_nodes, dtype=dtype, device=self.device)
+        return torch.zeros(self.n_nodes, 4, dtype=dtype, device=self.device)
 
     def new_sort(self):
         return torch.zeros(self.sortlen, dtype=torch.int32, device=self.device)
 
     # ------------------------------------------------------------ packing
+    # ``perm`` (optional, per instance): slot j of instance i holds particle
+    # perm[i][j] of the caller's arrays.  ``cell_order`` gives the order that
+    # sorts particles by bin key, so a kernel's neighbouring lanes touch
+    # neighbouring grid nodes and the sorted payload writes are near-coalesced.
     def sl(self, i):
         return slice(self.p_off[i], self.p_off[i] + self.specs[i].n)
 
-    def pack_dyn(self, parts, dtype):
+    def cell_order(self, pts_list):
+        perms = []
+        for i, pts in enumerate(pts_list):
+            s = self.specs[i]
+            x = np.asarray(pts.detach().cpu() if isinstance(pts, torch.Tensor) else pts,
+                           dtype=np.float64)
+            b = np.floor(x * (1.0 / s.dx) - 0.5).astype(np.int64) + 2
+            dims = np.array(s.shape) + 2
+            b = np.clip(b, 0, dims - 1)
+            T = np.array(self.tile)
+            sh = np.array([self._insts[i].shift[a] for a in range(3)])
+            b = b + sh
+            t = b // T
+            nt = -(-(dims + sh) // T)
+            tile_id = (t[:, 0] * nt[1] + t[:, 1]) * nt[2] + t[:, 2]
+            l = b - t * T
+            local = (l[:, 0] * T[1] + l[:, 1]) * T[2] + l[:, 2]
+            perms.append(np.argsort(tile_id * int(T.prod()) + local, kind="stable"))
+        return perms
+
+    def _cols(self, a, i, n, k, dtype, perm):
+        t = _dev(a, dtype, self.device).reshape(n, k)
+        if perm is not None and perm[i] is not None:
+            t = t[torch.as_tensor(perm[i], device=self.device)]
+        return t.T
+
+    def pack_dyn(self, parts, dtype, perm=None):
         """parts: list over instances of (x, v, C, F) with shapes (n,3),(n,3),(n,3,3),(n,3,3)."""
         dyn = self.new_dyn(dtype)
         for i, (x, v, C, F) in enumerate(parts):
             s = self.sl(i)
             n = self.specs[i].n
-            dyn[ROW_X:ROW_X + 3, s] = _dev(x, dtype, self.device).reshape(n, 3).T
-            dyn[ROW_V:ROW_V + 3, s] = _dev(v, dtype, self.device).reshape(n, 3).T
-            dyn[ROW_C:ROW_C + 9, s] = _dev(C, dtype, self.device).reshape(n, 9).T
-            dyn[ROW_F:ROW_F + 9, s] = _dev(F, dtype, self.device).reshape(n, 9).T
+            dyn[ROW_X:ROW_X + 3, s] = self._cols(x, i, n, 3, dtype, perm)
+            dyn[ROW_V:ROW_V + 3, s] = self._cols(v, i, n, 3, dtype, perm)
+            dyn[ROW_C:ROW_C + 9, s] = self._cols(C, i, n, 9, dtype, perm)
+            dyn[ROW_F:ROW_F + 9, s] = self._cols(F, i, n, 9, dtype, perm)
         return dyn
 
-    def pack_mat(self, mats, dtype):
+    def pack_mat(self, mats, dtype, perm=None):
         out = self.new_mat(dtype)
         for i, M in enumerate(mats):
-            n = self.specs[i].n
-            out[:, self.sl(i)] = _dev(M, dtype, self.device).reshape(n, 9).T
+            out[:, self.sl(i)] = self._cols(M, i, self.specs[i].n, 9, dtype, perm)
         return out
 
-    def unpack_dyn(self, dyn, i=0):
+    def _uncols(self, t, i, perm):
+        if perm is None or perm[i] is None:
+            return t
+        inv = torch.empty(len(perm[i]), dtype=torch.int64, device=self.device)
+        inv[torch.as_tensor(perm[i], device=self.device)] = torch.arange(len(perm[i]),
+                                                                          device=self.device)
+        return t[inv]
+
+    def unpack_dyn(self, dyn, i=0, perm=None):
         s = self.sl(i)
         n = self.specs[i].n
-        return (dyn[ROW_X:ROW_X + 3, s].T, dyn[ROW_V:ROW_V + 3, s].T,
-                dyn[ROW_C:ROW_C + 9, s].T.reshape(n, 3, 3), dyn[ROW_F:ROW_F + 9, s].T.reshape(n, 3, 3))
+        u = lambda t: self._uncols(t, i, perm)
+        return (u(dyn[ROW_X:ROW_X + 3, s].T), u(dyn[ROW_V:ROW_V + 3, s].T),
+                u(dyn[ROW_C:ROW_C + 9, s].T).reshape(n, 3, 3),
+                u(dyn[ROW_F:ROW_F + 9, s].T).reshape(n, 3, 3))
 
-    def unpack_mat(self, M, i=0):
+    def unpack_mat(self, M, i=0, perm=None):
         n = self.specs[i].n
-        return M[:, self.sl(i)].T.reshape(n, 3, 3)
+        return self._uncols(M[:, self.sl(i)].T, i, perm).reshape(n, 3, 3)
 
     def per_instance_sum(self, v):
         """Sum of a per-slot float64 vector over each instance's live slots, in a
@@ -259,7 +311,7 @@ class Trajectory:
         self.ckpt = torch.zeros(max(1, self.nseg), N_DYN, p.n_slots, dtype=dtype, device=p.device)
         Ks = self.K
         self.seg_dyn = torch.zeros(Ks, N_DYN, p.n_slots, dtype=dtype, device=p.device)
-        self.seg_grid = torch.zeros(Ks, 4, p.n_nodes, dtype=dtype, device=p.device)
+        self.seg_grid = torch.zeros(Ks, p.n_nodes, 4, dtype=dtype, device=p.device)
         self.seg_sort = torch.zeros(Ks, p.sortlen, dtype=torch.int32, device=p.device)
         self.G = p.new_mat(dtype)
         self.bar_a = p.new_dyn(dtype)

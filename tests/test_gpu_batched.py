@@ -1,0 +1,152 @@
+"""Batched launches: many instances of uneven size in one plan, against the
+FP64 oracle per instance and against single-instance runs."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ref_loop as RL
+from oracle import ref_vec as RV
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_model(m):
+    if m.kind == "hb":
+        return RL.HerschelBulkley(m.kappa, m.K, m.n, m.tau_y, m.m, m.eps)
+    mat = RL.Material()
+    o = RL.NeoHookean(mat) if m.kind == "nh" else RL.Hencky(mat)
+    o.mu, o.lam = m.mu, m.lam
+    return o
+
+
+def _oracle_cfg(cfg):
+    return RL.SimConfig(N=cfg.N, dx=cfg.dx, dt=cfg.dt, damping=cfg.damping, sp=cfg.sp,
+                        material=RL.Material(cfg.material.E, cfg.material.nu,
+                                             cfg.material.density), dims=cfg.dims)
+
+
+def _small_batch(models=("nh", "hencky", "nh", "hencky"), n_steps=12, seed=1):
+    from morphompm_amd import constitutive as K
+    from morphompm_amd.scenes import batch_instances
+    sc = batch_instances(count=len(models), seed=seed, n_lo=200, n_hi=900, n_steps=n_steps)
+    for s, m in zip(sc, models):
+        mat = s.cfg.material
+        s.model = {"nh": K.NeoHookean(mat), "hencky": K.Hencky(mat),
+                   "hb": K.HerschelBulkley(kappa=mat.lam + 2 * mat.mu / 3, K=10.0, n=0.6)}[m]
+        s.checkpoint_every = 5
+    return sc
+
+
+def _run_batch(scenes, dtype, sched=True):
+    from morphompm_amd.batched import BatchedInverse
+    bi = BatchedInverse(scenes, dtype=dtype, use_graphs=False)
+    bi.set_theta([s.extra["rate"] for s in scenes])
+    L, g = bi.loss_and_grad()
+    return bi, L.cpu().numpy(), g.cpu().numpy()
+
+
+def _oracle_instance(s, rate):
+    cfg = _oracle_cfg(s.cfg)
+    m = _oracle_model(s.model)
+    st0 = RL.ParticleState.rest(s.pts)
+    st0.Fg = np.eye(3) + rate * s.dG
+    fin, caches = RV.rollout(st0, m, cfg, s.n_steps, advect=True, sched=s.sched)
+    det = np.linalg.det(fin.F)
+    n = s.n
+    Fb = det[:, None, None] * np.swapaxes(np.linalg.inv(fin.F), 1, 2)
+    z3, z33 = np.zeros((n, 3)), np.zeros((n, 3, 3))
+    Gb = RV.rollout_vjp(caches, z3, z3, z33, Fb, z33, advect=True, sched=s.sched)[4]
+    return fin, det.sum(), float(np.sum(Gb * s.dG))
+
+
+@pytest.mark.parametrize("models", [("nh", "hencky", "nh", "hencky"),
+                                    ("hencky", "hencky", "hencky"),
+                                    ("nh", "hb", "hencky")])
+def test_batched_fp64_matches_oracle(models):
+    sc = _small_batch(models)
+    bi, L, g = _run_batch(sc, torch.float64)
+    p = bi.plan
+    for i, s in enumerate(sc):
+        fin, Lo, go = _oracle_instance(s, s.extra["rate"])
+        x, v, C, F = (t.cpu().numpy() for t in p.unpack_dyn(bi.final_state(), i, bi.perm))
+        assert np.max(np.abs(F - fin.F)) <= 1e-10 * np.max(np.abs(fin.F)), i
+        assert np.max(np.abs(x - fin.x)) <= 1e-12, i
+        assert abs(L[i] - Lo) <= 1e-12 * abs(Lo), i
+        assert abs(g[i] - go) <= 1e-9 * abs(go), (i, g[i], go)
+
+
+def test_batched_fp32_error_vs_fp64():
+    sc = _small_batch(("nh", "hencky", "nh", "hencky"), n_steps=30)
+    _, L32, g32 = _run_batch(sc, torch.float32)
+    _, L64, g64 = _run_batch(sc, torch.float64)
+    assert np.max(np.abs(L32 - L64) / np.abs(L64)) < 1e-5
+    assert np.max(np.abs(g32 - g64) / np.abs(g64)) < 1e-5
+
+
+def test_batched_equals_single_instance_bitwise():
+    """Each instance in a batch gives exactly the bits it gives alone (same
+    slot order and tile alignment): instances never interact."""
+    from morphompm_amd.engine import BatchPlan, InstanceSpec, Trajectory
+    sc = _small_batch(("nh", "hencky", "nh"), n_steps=8)
+    bi, _, _ = _run_batch(sc, torch.float32)
+    for i, s in enumerate(sc):
+        p = BatchPlan([InstanceSpec.from_cfg(s.n, s.cfg, s.model, s.pts)])
+        perm = p.cell_order([s.pts])
+        tr = Trajectory(p, torch.float32, s.n_steps, True, 5, s.sched)
+        dyn0 = p.pack_dyn([(s.pts, np.zeros((s.n, 3)), np.zeros((s.n, 3, 3)),
+                            np.repeat(np.eye(3)[None], s.n, 0))], torch.float32, perm)
+        G = p.pack_mat([np.eye(3) + s.extra["rate"] * s.dG], torch.float32, perm)
+        fin = tr.forward(dyn0, G)
+        a = [t.cpu() for t in p.unpack_dyn(fin, 0, perm)]
+        b = [t.cpu() for t in bi.plan.unpack_dyn(bi.final_state(), i, bi.perm)]
+        for u, w in zip(a, b):
+            assert torch.equal(u, w), i
+
+
+def test_graph_replays_match_eager():
+    """Captured forward/adjoint graphs replayed several times, with eager work
+    in between, reproduce the eager result every time."""
+    from morphompm_amd.batched import BatchedInverse
+    sc = _small_batch(("nh", "hencky"), n_steps=10)
+    eager = BatchedInverse(sc, dtype=torch.float32, use_graphs=False)
+    graph = BatchedInverse(sc, dtype=torch.float32, use_graphs=True)
+    rates = np.array([s.extra["rate"] for s in sc])
+    for rep in range(3):
+        th = rates * (1.0 + 0.1 * rep)
+        eager.set_theta(th)
+        graph.set_theta(th)
+        Le, ge = eager.loss_and_grad()
+        Lg, gg = graph.loss_and_grad()
+        torch.cuda.synchronize()
+        assert torch.equal(Le, Lg) and torch.equal(ge, gg), rep
+
+
+def test_hb_theta_batched_fd_fp64():
+    """Herschel-Bulkley (tau_y, K) cotangents from the batched adjoint against
+    central differences of the FP64 GPU forward."""
+    from morphompm_amd import constitutive as K
+    from morphompm_amd.engine import BatchPlan, InstanceSpec, Trajectory, ROW_F
+    from morphompm_amd.batched import cofactor_rows
+    from morphompm_amd.scenes import free_swell_cube
+    sc = free_swell_cube(cells=4, origin=6, N=16, n_steps=6, dt=1e-4)
+    mat = sc.cfg.material
+
+    def run(ty, Kc, want_grad):
+        m = K.HerschelBulkley(kappa=mat.lam + 2 * mat.mu / 3, K=Kc, n=0.6, tau_y=ty)
+        p = BatchPlan([InstanceSpec.from_cfg(sc.n, sc.cfg, m, sc.pts)])
+        tr = Trajectory(p, torch.float64, sc.n_steps, True, 3, sc.sched)
+        dyn0 = p.pack_dyn([(sc.pts, np.zeros((sc.n, 3)), np.zeros((sc.n, 3, 3)),
+                            np.repeat(np.eye(3)[None], sc.n, 0))], torch.float64)
+        fin = tr.forward(dyn0, p.pack_mat([sc.G], torch.float64))
+        det, cof = cofactor_rows(fin[ROW_F:ROW_F + 9])
+        L = float((det * p.live).sum())
+        if not want_grad:
+            return L
+        bar = torch.zeros_like(fin)
+        bar[ROW_F:ROW_F + 9] = cof * p.live
+        _, _, th = tr.backward(bar, want_theta=True)
+        return L, th[0].cpu().numpy()
+    _, th = run(50.0, 10.0, True)
+    for k, (dty, dK) in enumerate([(1e-3, 0.0), (0.0, 1e-4)]):
+        fd = (run(50.0 + dty, 10.0 + dK, False) - run(50.0 - dty, 10.0 - dK, False)) / (2 * (dty + dK))
+        assert abs(th[k] - fd) <= 1e-6 * abs(fd) + 1e-14, (k, th[k], fd)
