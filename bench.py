@@ -27,6 +27,23 @@ sys.path.insert(0, ROOT)
 METRIC = ("fwd+adjoint particle-steps/s, batched FP32 vs plain GPU, grad rel.err≤1e-5")
 HBM_PEAK_GBS = 8000.0
 
+# Kernel ids of the library's timer (include/morphompm_hip.h) with each
+# kernel's compulsory bytes per launch (FP32): n particles, a active grid
+# nodes, nb bins.  Per particle: records of 4-byte words read + written.
+KERNELS = {
+    0: ("sort", lambda n, a, nb: 20 * n + 4 * nb),          # x in, key/rank/sorted; bin counts
+    1: ("prep_fwd", lambda n, a, nb: 196 * n),              # x v C F G in, 16-word payload out
+    2: ("p2g_fwd_tile", lambda n, a, nb: 64 * n + 16 * a),  # payload in, node partials out
+    3: ("grid_fwd", lambda n, a, nb: 32 * a),               # partials in, (m, v) out
+    4: ("g2p", lambda n, a, nb: 144 * n + 16 * a),          # x F + node (m, v) in, x v C F out
+    5: ("prep_adj", lambda n, a, nb: 208 * n),              # x F + 4 cotangents in, payload out
+    6: ("p2g_adj_tile", lambda n, a, nb: 64 * n + 16 * a),
+    7: ("grid_adj", lambda n, a, nb: 48 * a),               # partials + grid in, adjoint grid out
+    8: ("adj_gather", lambda n, a, nb: 200 * n + 32 * a),   # 2 payloads + xn_b + nodes in, x_b v_b MD out
+    9: ("adj_vjp", lambda n, a, nb: 360 * n),               # MD C F G Cn Fn_b G_b in, C_b F_b G_b out
+}
+ADJ_KERNELS = {5, 6, 7, 8, 9}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -42,9 +59,6 @@ def parse():
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--tile", default="4,8,8")
     ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--timer", choices=("graph", "eager"), default="eager",
-                    help="time the dominant kernel inside the captured graph replays of the "
-                         "timed region, or in an eager instrumented iteration after it")
     return ap.parse_args()
 
 
@@ -133,37 +147,6 @@ def main():
     target = target.cpu().numpy()
     theta = 0.8 * rates
 
-    # dominant kernel timer, armed for the capture of the adjoint graph
-    kid = 8  # MPM_K_P2G_ADJ
-    n_launch = a.sim_steps
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * n_launch)]
-    for e in evs:
-        e.record()
-    torch.cuda.synchronize()
-    handles = (ctypes.c_void_p * (2 * n_launch))(*[e.cuda_event for e in evs])
-    L = _lib.lib()
-    L.mpm_kernel_timer.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]
-    bi._g_fwd = bi._g_bwd = None
-
-    def capture_with_timer():
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            bi._fwd()
-            bi._bwd()
-        torch.cuda.current_stream().wait_stream(s)
-        torch.cuda.synchronize()
-        bi._g_fwd = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(bi._g_fwd):
-            bi._fwd()
-        bi._g_bwd = torch.cuda.CUDAGraph()
-        L.mpm_kernel_timer(kid, handles, n_launch)
-        with torch.cuda.graph(bi._g_bwd):
-            bi._bwd()
-        L.mpm_kernel_timer(-1, None, 0)
-    if a.timer == "graph":
-        bi._capture = capture_with_timer
-
     def gn_iteration(theta):
         bi.set_theta(theta)
         Lv, dL = (t.cpu().numpy() for t in bi.loss_and_grad())
@@ -193,32 +176,50 @@ def main():
     print(f"[bench] {ms_per:.2f} ms per iteration, {value:.4g} particle-steps/s "
           f"(wall {wall:.2f} s)", file=sys.stderr, flush=True)
 
-    if a.timer == "eager":
-        # one more iteration, eager, with the dominant kernel's launches timed
-        bi.set_theta(theta)
+    # Per-kernel timing: one more iteration per kernel, run eagerly on the
+    # same stream with HIP events around each launch of that kernel (event
+    # records inside captured graphs are refused on this stack).
+    import ctypes
+    L = _lib.lib()
+    cap = 2 * a.sim_steps
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * cap)]
+    for e in evs:
+        e.record()
+    torch.cuda.synchronize()
+    handles = (ctypes.c_void_p * (2 * cap))(*[e.cuda_event for e in evs])
+    bi.set_theta(theta)
+    ktab = {}
+    for kid, (kname, _) in KERNELS.items():
+        L.mpm_kernel_timer(kid, handles, cap)
         bi._fwd()
-        L.mpm_kernel_timer(kid, handles, n_launch)
         bi._bwd()
+        cnt = L.mpm_kernel_timer_count()
         L.mpm_kernel_timer(-1, None, 0)
         torch.cuda.synchronize()
-    # dominant kernel: average launch duration over one iteration's launches
-    durs = [evs[2 * i].elapsed_time(evs[2 * i + 1]) for i in range(n_launch)]
-    avg_ms = float(np.mean(durs))
-    grid = bi.traj.seg_grid[0]
-    active = int((grid[:, 0] > 1e-12).sum())
-    bytes_launch = 432.0 * n_total + 28.0 * active
-    achieved = bytes_launch / (avg_ms / 1e3) / 1e9
+        d = [evs[2 * i].elapsed_time(evs[2 * i + 1]) for i in range(cnt)]
+        ktab[kname] = {"id": kid, "avg_ms": float(np.mean(d)), "launches_timed": cnt}
+    grids = bi.traj.seg_grid
+    active = float(np.mean([int((grids[k][:, 0] > 1e-12).sum()) for k in range(grids.shape[0])]))
+    iter_launch = {k: (a.sim_steps if k in ADJ_KERNELS else
+                       a.sim_steps + (a.sim_steps - bi.traj._seg(bi.traj.nseg - 1)[1]))
+                   for k in KERNELS}
+    for kname, e in ktab.items():
+        kid = e["id"]
+        e["algorithmic_bytes_per_launch"] = KERNELS[kid][1](n_total, active, bi.plan.n_bins)
+        e["achieved_GBs"] = e["algorithmic_bytes_per_launch"] / (e["avg_ms"] / 1e3) / 1e9
+        e["ms_per_iteration"] = e["avg_ms"] * iter_launch[kid]
+    dom = max(ktab, key=lambda k: ktab[k]["ms_per_iteration"])
+    d = ktab[dom]
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_p2g_adj_batched.txt")
+    tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_batched.txt")
     if os.path.exists(tf):
         for line in open(tf):
             if line.startswith("traffic_bytes_per_launch"):
                 traffic = float(line.split()[1])
-    roof = {"kernel": "k_p2g_adj (P2G-adjoint gather + constitutive VJP)", "bound": "hbm",
-            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_launch,
-            "launches_timed": n_launch, "timer": a.timer}
+    roof = {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+            "avg_launch_ms": d["avg_ms"], "algorithmic_bytes_per_launch": d["algorithmic_bytes_per_launch"],
+            "launches_timed": d["launches_timed"], "timer": "hip events, eager iteration after the timed region"}
 
     # whole-iteration algorithmic bytes (fwd + adjoint, per particle-step)
     r = active / n_total
@@ -291,7 +292,8 @@ def main():
                       "instances": len(scenes), "particles": n_total,
                       "sim_steps": a.sim_steps, "checkpoint_every": a.checkpoint_every,
                       "tile": list(tile), "block": a.block, "hip_graphs": True},
-           "roofline": roof, "whole_step_roofline": whole, "cpu_baseline": cpu,
+           "roofline": roof, "whole_step_roofline": whole, "kernels": ktab,
+           "cpu_baseline": cpu,
            "plain_gpu": plain,
            "batched_over_plain": (value / plain["per_instance_value"]) if plain else None,
            "batched_over_cpu": (value / cpu["value"]) if cpu else None,

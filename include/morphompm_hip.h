@@ -20,6 +20,13 @@
  * Grid layout: grid [n_nodes][4]: per node (mass, vx, vy, vz).
  * Sort record for one step: sortbuf int32 [2 n_slots + n_bins + 1]:
  *   rank[n_slots] | sorted[n_slots] | bin_start[n_bins + 1]
+ *   (each instance's sorted range is its own slot range; padding slots sort
+ *   into the instance's dead bin)
+ * P2G payload for one step: payload [n_slots][16] in sorted order, per
+ *   particle (fx[3], q[3], A'[9], cell) -- the forward step's momentum map
+ *   m_i += w pmass, mom_i += w (q + A' o) for stencil offset o, and the packed
+ *   cell ((base+2) per axis in 10 bits, all ones off the grid).  Grids are
+ *   limited to 1021 nodes per axis by this packing.
  *
  * Reference interfaces these replace (paths in 2nugu/morphompm):
  *   python/morphompm/constitutive.py:1-12 (stress / stress_vjp seam),
@@ -123,17 +130,17 @@ int32_t mpm_sort_by_cell_f64(const mpm_batch* b, const double* dyn, int32_t* sor
 
 /* One forward step: sort, P2G (deterministic, no float atomics), grid update,
  * G2P.  Replaces transfer.step (python/morphompm/transfer.py:33-79); writes
- * dyn_out, grid and sortbuf (the latter two are what step_bwd needs). */
+ * dyn_out, and grid, sortbuf and payload, which step_bwd needs. */
 int32_t mpm_step_fwd_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
                          const float* sched, float* dyn_out, float* grid, int32_t* sortbuf,
-                         int32_t advect, void* ws, void* stream);
+                         float* payload, int32_t advect, void* ws, void* stream);
 int32_t mpm_step_fwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
                          const double* sched, double* dyn_out, double* grid, int32_t* sortbuf,
-                         int32_t advect, void* ws, void* stream);
+                         double* payload, int32_t advect, void* ws, void* stream);
 
 /* Reverse of one step.  Replaces transfer.step_vjp
  * (python/morphompm/transfer.py:82-158).  Inputs are the forward step's
- * dyn_in, Fg, sched, dyn_out, grid and sortbuf, and the cotangents of the
+ * dyn_in, Fg, sched, dyn_out, grid, sortbuf and payload, and the cotangents of the
  * step's outputs bar_out [24][n_slots].  Writes bar_in [24][n_slots] and adds
  * a * (local Fg cotangent) into Fg_bar [9][n_slots] (the reference's
  * pass-through Fg_b = Fgn_b + local when a = 1).  theta_acc (may be NULL)
@@ -141,14 +148,14 @@ int32_t mpm_step_fwd_f64(const mpm_batch* b, const double* dyn_in, const double*
  * [n_slots/256][2] float64. */
 int32_t mpm_step_bwd_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
                          const float* sched, const float* dyn_out, const float* grid,
-                         const int32_t* sortbuf, const float* bar_out, float* bar_in,
-                         float* Fg_bar, double* theta_acc, int32_t advect, void* ws,
+                         const int32_t* sortbuf, const float* payload, const float* bar_out,
+                         float* bar_in, float* Fg_bar, double* theta_acc, int32_t advect, void* ws,
                          void* stream);
 int32_t mpm_step_bwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
                          const double* sched, const double* dyn_out, const double* grid,
-                         const int32_t* sortbuf, const double* bar_out, double* bar_in,
-                         double* Fg_bar, double* theta_acc, int32_t advect, void* ws,
-                         void* stream);
+                         const int32_t* sortbuf, const double* payload, const double* bar_out,
+                         double* bar_in, double* Fg_bar, double* theta_acc, int32_t advect,
+                         void* ws, void* stream);
 
 /* Per-launch timing of one kernel (profiling aid, no reference counterpart).
  * `events` holds 2*n_pairs hipEvent_t handles created with timing enabled.
@@ -166,7 +173,8 @@ enum {
   MPM_K_PREP_ADJ = 5,
   MPM_K_P2G_ADJ_TILE = 6, /* G2P-adjoint scatter as a tile reduction */
   MPM_K_GRID_ADJ = 7,
-  MPM_K_P2G_ADJ = 8      /* P2G-adjoint gather + constitutive VJP */
+  MPM_K_P2G_ADJ = 8,     /* P2G-adjoint gather (x, v cotangents, MD) */
+  MPM_K_ADJ_VJP = 9      /* constitutive VJP, F, C, Fg cotangents */
 };
 int32_t mpm_kernel_timer(int32_t kernel_id, void* const* events, int32_t n_pairs);
 int32_t mpm_kernel_timer_count(void);

@@ -70,10 +70,13 @@ def lib():
         f.argtypes = [ctypes.POINTER(MpmBatch), _P, _P, _P, _P]
         f = getattr(L, f"mpm_step_fwd_{suf}")
         f.restype = _I32
-        f.argtypes = [ctypes.POINTER(MpmBatch), _P, _P, _P, _P, _P, _P, _I32, _P, _P]
+        f.argtypes = [ctypes.POINTER(MpmBatch), _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P]
         f = getattr(L, f"mpm_step_bwd_{suf}")
         f.restype = _I32
-        f.argtypes = [ctypes.POINTER(MpmBatch)] + [_P] * 10 + [_I32, _P, _P]
+        f.argtypes = [ctypes.POINTER(MpmBatch)] + [_P] * 11 + [_I32, _P, _P]
+    L.mpm_kernel_timer.restype = _I32
+    L.mpm_kernel_timer.argtypes = [_I32, _P, _I32]
+    L.mpm_kernel_timer_count.restype = _I32
     if L.mpm_abi_version() != ABI_VERSION:
         raise MpmError(f"ABI mismatch: library {L.mpm_abi_version()} vs binding {ABI_VERSION}")
     _lib = L

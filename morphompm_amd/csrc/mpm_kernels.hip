@@ -196,17 +196,28 @@ __global__ __launch_bounds__(256) void k_sort_key(const mpm_instance* inst, cons
                                                   int32_t* count) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const mpm_instance& I = inst[block_inst(slot_inst)];
+  // Padding slots belong to the dead bin too, so each instance's sorted range
+  // is exactly its own slot range [p_off, p_off + padded count): they take the
+  // tail of that range in slot order (k_place) and no atomic slot; the first
+  // padding slot adds their count to the dead bin once.
   int64_t k = -1;
-  if (p < n_slots && p - I.p_off < I.n) {
-    T x[3], fx[3];
-    int base[3];
-    ld3(dyn, n_slots, p, RX, x);
-    stencil_base(x, (T)(1.0 / I.dx), base, fx);
-    const int bx = base[0] + 2, by = base[1] + 2, bz = base[2] + 2;
-    if (bx >= 0 && bx < I.nx + 2 && by >= 0 && by < I.ny + 2 && bz >= 0 && bz < I.nz + 2)
-      k = bin_key(I, bx, by, bz, TX, TY, TZ);
-    else
-      k = I.c_off + (int64_t)I.tx * I.ty * I.tz * (TX * TY * TZ);  // dead bin
+  if (p < n_slots) {
+    const int64_t dead = I.c_off + (int64_t)I.tx * I.ty * I.tz * (TX * TY * TZ);
+    const int64_t local = p - I.p_off;
+    if (local == I.n) {
+      const int64_t padded = ((int64_t)I.n + kChunk - 1) / kChunk * kChunk;
+      atomicAdd(&count[dead], (int32_t)((padded > 0 ? padded : kChunk) - I.n));
+    }
+    if (local < I.n) {
+      k = dead;
+      T x[3], fx[3];
+      int base[3];
+      ld3(dyn, n_slots, p, RX, x);
+      stencil_base(x, (T)(1.0 / I.dx), base, fx);
+      const int bx = base[0] + 2, by = base[1] + 2, bz = base[2] + 2;
+      if (bx >= 0 && bx < I.nx + 2 && by >= 0 && by < I.ny + 2 && bz >= 0 && bz < I.nz + 2)
+        k = bin_key(I, bx, by, bz, TX, TY, TZ);
+    }
   }
   // One atomic per run of equal keys in adjacent lanes (particles arrive
   // nearly sorted); the per-bin sort afterwards fixes the order in a bin.
@@ -286,15 +297,23 @@ __global__ void k_place(const int32_t* key, const int32_t* slot, const int32_t* 
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_slots) return;
   const int32_t k = key[p];
-  if (k < 0) return;
+  if (k < 0) {
+    sorted[p] = (int32_t)p;  // padding slot: the tail of its instance's range, in order
+    return;
+  }
   sorted[start[k] + slot[p]] = (int32_t)p;
 }
 
-// Insertion sort of each bin by slot (ascending), then rank[sorted[j]] = j.
+// Each bin sorted by slot (ascending), then rank[sorted[j]] = j.  Bins whose
+// entries already ascend (one run of adjacent lanes) are only checked, with
+// independent loads; the rest get an insertion sort.
 __global__ void k_bin_sort(const int32_t* start, int32_t* sorted, int32_t* rank, int64_t n_bins) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n_bins) return;
   const int32_t s0 = start[b], s1 = start[b + 1];
+  bool ascending = true;
+  for (int32_t j = s0 + 1; j < s1; ++j) ascending &= sorted[j - 1] < sorted[j];
+  if (!ascending)
   for (int32_t j = s0 + 1; j < s1; ++j) {
     const int32_t v = sorted[j];
     int32_t i = j - 1;
@@ -307,12 +326,28 @@ __global__ void k_bin_sort(const int32_t* start, int32_t* sorted, int32_t* rank,
   for (int32_t j = s0; j < s1; ++j) rank[sorted[j]] = j;
 }
 
+// Record slot 15 carries the particle's cell as bits: (base+2) per axis in 10
+// bits each, or all ones when no stencil node is in the grid.
+constexpr uint32_t kNoCell = 0xffffffffu;
+__device__ __forceinline__ uint32_t pack_cell(const int base[3], int nx, int ny, int nz) {
+  const int bx = base[0] + 2, by = base[1] + 2, bz = base[2] + 2;
+  if (bx < 0 || bx >= nx + 2 || by < 0 || by >= ny + 2 || bz < 0 || bz >= nz + 2) return kNoCell;
+  return (uint32_t)bx | ((uint32_t)by << 10) | ((uint32_t)bz << 20);
+}
+__device__ __forceinline__ float cell_as(float, uint32_t c) { return __uint_as_float(c); }
+__device__ __forceinline__ double cell_as(double, uint32_t c) {
+  return __longlong_as_double((long long)c);
+}
+__device__ __forceinline__ uint32_t cell_of(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ uint32_t cell_of(double v) { return (uint32_t)__double_as_longlong(v); }
+
 template <typename T>
-__device__ __forceinline__ void store_payload(T* r, const T fx[3], const T q[3], const M3<T>& A) {
+__device__ __forceinline__ void store_payload(T* r, const T fx[3], const T q[3], const M3<T>& A,
+                                              uint32_t cell) {
   st4(r, fx[0], fx[1], fx[2], q[0]);
   st4(r + 4, q[1], q[2], A.a[0], A.a[1]);
   st4(r + 8, A.a[2], A.a[3], A.a[4], A.a[5]);
-  st4(r + 12, A.a[6], A.a[7], A.a[8], T(0));
+  st4(r + 12, A.a[6], A.a[7], A.a[8], cell_as(T(0), cell));
 }
 
 // ---------------------------------------------------------------------------
@@ -345,7 +380,8 @@ __global__ __launch_bounds__(256) void k_prep_fwd(const mpm_instance* inst, cons
   T q[3];
   for (int a = 0; a < 3; ++a)
     q[a] = I.pmass * v[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
-  store_payload(payload + (int64_t)rank[p] * kRecord, fx, q, Ap);
+  store_payload(payload + (int64_t)rank[p] * kRecord, fx, q, Ap,
+                pack_cell(base, I.nx, I.ny, I.nz));
 }
 
 // ---------------------------------------------------------------------------
@@ -561,7 +597,8 @@ __global__ __launch_bounds__(256) void k_prep_adj(const mpm_instance* inst, cons
   T q[3];
   for (int a = 0; a < 3; ++a)
     q[a] = nvb[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
-  store_payload(payload + (int64_t)rank[p] * kRecord, fx, q, Ap);
+  store_payload(payload + (int64_t)rank[p] * kRecord, fx, q, Ap,
+                pack_cell(base, I.nx, I.ny, I.nz));
 }
 
 // reverse step, part 2 (transfer.py:121-127): grid adjoint.
@@ -591,67 +628,51 @@ __global__ __launch_bounds__(256) void k_grid_adj(const mpm_instance* inst, cons
   st4(gadj + 4 * g, gmb, mb[0], mb[1], mb[2]);
 }
 
-// reverse step, part 3 (transfer.py:100-119 x_bar terms, :129-156):
-// P2G-adjoint gather, advection terms, constitutive VJP, Fg and theta paths.
-//
-// With dpos = (o - fx) dx for stencil offset o, the per-node factors are
-// affine in o and are built incrementally:
-//   nv_b + 4 inv_dx^2 C_b' dpos = q1 + P1 o,   P1 = 4 inv_dx^2 dx C_b',  q1 = nv_b - P1 fx
-//   pmass v + A dpos            = q2 + A' o,   A' = A dx,                  q2 = pmass v - A' fx
+// reverse step, part 3a (transfer.py:100-119 x_bar terms, :129-147): the
+// P2G-adjoint gather, one thread per sorted slot j (neighbouring lanes share
+// stencil nodes).  Reads the forward payload pf (fx, q2, A', cell) and the
+// adjoint payload pa (fx, q1, P1) of the particle in slot j.  With dpos =
+// (o - fx) dx for stencil offset o, the per-node factors are affine in o:
+//   nv_b + 4 inv_dx^2 C_b' dpos = q1 + P1 o   (P1 = 4 inv_dx^2 dx C_b')
+//   pmass v + A dpos            = q2 + A' o   (A' = A dx, A = pmass C + dt c tau)
 // so x_bar's grad-w coefficient is v_i.(q1 + P1 o) + mb_i.(q2 + A' o) + gm_b,i pmass.
-// F, Fg and C are reloaded after the node loop rather than kept live in it.
-template <typename T, int MODEL>
-__global__ __launch_bounds__(256) void k_p2g_adj(const mpm_instance* inst, const int32_t* slot_inst,
-                                                 int64_t n_slots, const T* dyn_in, const T* Fgp,
-                                                 const T* sched, const T* dyn_out, const T* grid,
-                                                 const T* gadj, const T* bar_out, T* bar_in,
-                                                 T* Fg_bar, double* theta_acc, int advect) {
-  __shared__ double red[2][256];
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int ii = block_inst(slot_inst);
-  const Inst<T> I = load_inst<T, MODEL>(inst, ii);
-  const bool live = (p < n_slots) && (p - I.p_off < I.n);
-  T th[2] = {T(0), T(0)};
-  if (live) {
-    T fx[3];
-    int base[3];
-    {
-      T x[3];
-      ld3(dyn_in, n_slots, p, RX, x);
-      stencil_base(x, I.inv_dx, base, fx);
-    }
-    const T a_t = sched ? sched[ii] : T(1);
-    T c, tau6[6];                 // tau symmetric: 00 01 02 11 12 22
-    M3<T> Ap, P1;
-    T q1[3], q2[3];
-    {
-      const M3<T> C = ld9(dyn_in, n_slots, p, RC);
-      const M3<T> F = ld9(dyn_in, n_slots, p, RF);
-      const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), a_t);
-      const M3<T> tau = kirchhoff(I.mp, F, Fg, C);
-      c = -I.k4 * I.vol0 * det(Fg);
-      tau6[0] = tau.a[0]; tau6[1] = tau.a[1]; tau6[2] = tau.a[2];
-      tau6[3] = tau.a[4]; tau6[4] = tau.a[5]; tau6[5] = tau.a[8];
-      for (int k = 0; k < 9; ++k) Ap.a[k] = (I.pmass * C.a[k] + I.dt * (c * tau.a[k])) * I.dx;
-      T v[3];
-      ld3(dyn_in, n_slots, p, RV, v);
-      for (int a = 0; a < 3; ++a)
-        q2[a] = I.pmass * v[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
-      const M3<T> Fnb = ld9(bar_out, n_slots, p, RF);
-      const M3<T> Cnb = ld9(bar_out, n_slots, p, RC);
-      const M3<T> FF = mul_nt(Fnb, F);
-      const T s1 = I.k4 * I.dx;
-      for (int k = 0; k < 9; ++k) P1.a[k] = s1 * (Cnb.a[k] + I.dt * FF.a[k]);
-      T nvb[3];
-      ld3(bar_out, n_slots, p, RV, nvb);
-      if (advect) {
-        T xnb[3];
-        ld3(bar_out, n_slots, p, RX, xnb);
-        for (int a = 0; a < 3; ++a) nvb[a] += I.dt * xnb[a];
-      }
-      for (int a = 0; a < 3; ++a)
-        q1[a] = nvb[a] - (P1(a, 0) * fx[0] + P1(a, 1) * fx[1] + P1(a, 2) * fx[2]);
-    }
+// Writes x_bar, v_bar of the particle and MD = sum w mb dpos^T (row-major, in
+// slot order) for the VJP kernel.
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_gather(const mpm_instance* inst,
+                                                    const int32_t* slot_inst, int64_t n_slots,
+                                                    const int32_t* sorted, const T* pf, const T* pa,
+                                                    const T* grid, const T* gadj, const T* bar_out,
+                                                    T* bar_in, T* md, int advect) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_slots) return;
+  const Inst<T> I = load_inst<T>(inst, block_inst(slot_inst));
+  const int64_t p = sorted[j];
+  if (p - I.p_off >= I.n) return;
+  const T* rf = pf + j * kRecord;
+  const T* ra = pa + j * kRecord;
+  T fx[3], q1[3], q2[3];
+  M3<T> P1, Ap;
+  uint32_t cell;
+  {
+    const V4<T> f0 = ld4(rf), f1 = ld4(rf + 4), f2 = ld4(rf + 8), f3 = ld4(rf + 12);
+    fx[0] = f0.x; fx[1] = f0.y; fx[2] = f0.z;
+    q2[0] = f0.w; q2[1] = f1.x; q2[2] = f1.y;
+    Ap.a[0] = f1.z; Ap.a[1] = f1.w; Ap.a[2] = f2.x; Ap.a[3] = f2.y; Ap.a[4] = f2.z;
+    Ap.a[5] = f2.w; Ap.a[6] = f3.x; Ap.a[7] = f3.y; Ap.a[8] = f3.z;
+    cell = cell_of(f3.w);
+    const V4<T> a0 = ld4(ra), a1 = ld4(ra + 4), a2 = ld4(ra + 8), a3 = ld4(ra + 12);
+    q1[0] = a0.w; q1[1] = a1.x; q1[2] = a1.y;
+    P1.a[0] = a1.z; P1.a[1] = a1.w; P1.a[2] = a2.x; P1.a[3] = a2.y; P1.a[4] = a2.z;
+    P1.a[5] = a2.w; P1.a[6] = a3.x; P1.a[7] = a3.y; P1.a[8] = a3.z;
+  }
+  T xb[3] = {T(0), T(0), T(0)};
+  T vb[3] = {T(0), T(0), T(0)};
+  M3<T> MD = zero3<T>();
+  T Sv[3] = {T(0), T(0), T(0)};  // sum w v_i
+  if (cell != kNoCell) {
+    const int base[3] = {(int)(cell & 1023u) - 2, (int)((cell >> 10) & 1023u) - 2,
+                         (int)(cell >> 20) - 2};
     const AxisNodes nx = axis_nodes(base[0], I.nx), ny = axis_nodes(base[1], I.ny),
                     nz = axis_nodes(base[2], I.nz);
     T w[3][3], dw[3][3];
@@ -659,25 +680,20 @@ __global__ __launch_bounds__(256) void k_p2g_adj(const mpm_instance* inst, const
       bspline_w(fx[a], w[a]);
       bspline_dw(fx[a], dw[a]);
     }
-    T xb[3] = {T(0), T(0), T(0)};
-    T vb[3] = {T(0), T(0), T(0)};
-    M3<T> MD = zero3<T>();
-    T Sv[3] = {T(0), T(0), T(0)};  // sum w v_i
     // The (i, j) loops stay rolled (9 rows of 3 nodes) to bound the gathered
-    // values in flight; per-axis values for row (i, j) are picked by select,
-    // not by dynamic register indexing.
+    // values in flight; per-axis values of row (i, j) are picked by select.
 #pragma unroll 1
     for (int ij = 0; ij < 9; ++ij) {
-      const int i = ij / 3, j = ij - 3 * (ij / 3);
+      const int i = ij / 3, jj = ij - 3 * (ij / 3);
       const T wxi = i == 0 ? w[0][0] : (i == 1 ? w[0][1] : w[0][2]);
       const T dxi = i == 0 ? dw[0][0] : (i == 1 ? dw[0][1] : dw[0][2]);
-      const T wyj = j == 0 ? w[1][0] : (j == 1 ? w[1][1] : w[1][2]);
-      const T dyj = j == 0 ? dw[1][0] : (j == 1 ? dw[1][1] : dw[1][2]);
+      const T wyj = jj == 0 ? w[1][0] : (jj == 1 ? w[1][1] : w[1][2]);
+      const T dyj = jj == 0 ? dw[1][0] : (jj == 1 ? dw[1][1] : dw[1][2]);
       const int nxi = i == 0 ? nx.idx[0] : (i == 1 ? nx.idx[1] : nx.idx[2]);
-      const int nyj = j == 0 ? ny.idx[0] : (j == 1 ? ny.idx[1] : ny.idx[2]);
+      const int nyj = jj == 0 ? ny.idx[0] : (jj == 1 ? ny.idx[1] : ny.idx[2]);
       const bool okxy = (i == 0 ? nx.ok[0] : (i == 1 ? nx.ok[1] : nx.ok[2])) &&
-                        (j == 0 ? ny.ok[0] : (j == 1 ? ny.ok[1] : ny.ok[2]));
-      const T Ti = T(i), Tj = T(j);
+                        (jj == 0 ? ny.ok[0] : (jj == 1 ? ny.ok[1] : ny.ok[2]));
+      const T Ti = T(i), Tj = T(jj);
       const T wij = wxi * wyj;
       const T d0 = (Ti - fx[0]) * I.dx, d1 = (Tj - fx[1]) * I.dx;
       T u1[3], u2[3];
@@ -714,34 +730,56 @@ __global__ __launch_bounds__(256) void k_p2g_adj(const mpm_instance* inst, const
         }
       }
     }
-    if (advect) {
-      // xn_b - (4 inv_dx^2 C_b')^T (sum w v_i) - A^T (sum w mb_i), A = A'/dx
-      T xnb[3];
-      ld3(bar_out, n_slots, p, RX, xnb);
-      for (int a = 0; a < 3; ++a)
-        xb[a] += xnb[a] - ((P1(0, a) * Sv[0] + P1(1, a) * Sv[1] + P1(2, a) * Sv[2]) +
-                           (Ap(0, a) * vb[0] + Ap(1, a) * vb[1] + Ap(2, a) * vb[2])) * I.inv_dx;
-    }
-    for (int a = 0; a < 3; ++a) vb[a] *= I.pmass;
-    st3(bar_in, n_slots, p, RX, xb);
-    st3(bar_in, n_slots, p, RV, vb);
-    // sk_b = dt MD, tau_b = c sk_b; grown-volume path <sk_b, tau>(-k4 vol0) det Fg Fg^{-T}
-    const M3<T> skb = scaled(MD, I.dt);
-    const T skt = skb.a[0] * tau6[0] + (skb.a[1] + skb.a[3]) * tau6[1] +
-                  (skb.a[2] + skb.a[6]) * tau6[2] + skb.a[4] * tau6[3] +
-                  (skb.a[5] + skb.a[7]) * tau6[4] + skb.a[8] * tau6[5];
-    const M3<T> C = ld9(dyn_in, n_slots, p, RC);
-    const M3<T> F = ld9(dyn_in, n_slots, p, RF);
+  }
+  if (advect) {
+    // x_bar += xn_b - (4 inv_dx^2 C_b')^T (sum w v_i) - A^T (sum w mb_i), A = A'/dx
+    T xnb[3];
+    ld3(bar_out, n_slots, p, RX, xnb);
+    for (int a = 0; a < 3; ++a)
+      xb[a] += xnb[a] - ((P1(0, a) * Sv[0] + P1(1, a) * Sv[1] + P1(2, a) * Sv[2]) +
+                         (Ap(0, a) * vb[0] + Ap(1, a) * vb[1] + Ap(2, a) * vb[2])) * I.inv_dx;
+  }
+  for (int a = 0; a < 3; ++a) vb[a] *= I.pmass;
+  st3(bar_in, n_slots, p, RX, xb);
+  st3(bar_in, n_slots, p, RV, vb);
+  st9(md, n_slots, j, 0, MD);
+}
+
+// reverse step, part 3b (transfer.py:148-156): constitutive VJP, one thread
+// per particle slot.  tau_b = c sk_b with sk_b = dt MD, c = -4 inv_dx^2 vol0 det Fg;
+//   C_bar = pmass MD + C_bar(stress),  F_bar = (I + dt Cn)^T Fn_b + F_bar(stress),
+//   G_bar += a (Fg_bar(stress) + <sk_b, tau> (-4 inv_dx^2 vol0) det Fg Fg^-T).
+template <typename T, int MODEL>
+__global__ __launch_bounds__(256) void k_adj_vjp(const mpm_instance* inst, const int32_t* slot_inst,
+                                                 int64_t n_slots, const int32_t* rank,
+                                                 const T* dyn_in, const T* Fgp, const T* sched,
+                                                 const T* dyn_out, const T* md, const T* bar_out,
+                                                 T* bar_in, T* Fg_bar, double* theta_acc) {
+  __shared__ double red[2][256];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ii = block_inst(slot_inst);
+  const Inst<T> I = load_inst<T, MODEL>(inst, ii);
+  const bool live = (p < n_slots) && (p - I.p_off < I.n);
+  T th[2] = {T(0), T(0)};
+  if (live) {
+    const M3<T> MD = ld9(md, n_slots, rank[p], 0);
+    const T a_t = sched ? sched[ii] : T(1);
     const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), a_t);
-    M3<T> Fpb, Fgb, Cbar;
-    kirchhoff_vjp(I.mp, F, Fg, C, scaled(skb, c), Fpb, Fgb, Cbar, th);
+    const T detG = det(Fg);
+    const T c = -I.k4 * I.vol0 * detG;
+    const M3<T> skb = scaled(MD, I.dt);
+    const M3<T> C = ld9(dyn_in, n_slots, p, RC);
+    M3<T> Fpb, Fgb, Cbar, tau;
+    {
+      const M3<T> F = ld9(dyn_in, n_slots, p, RF);
+      kirchhoff_vjp(I.mp, F, Fg, C, scaled(skb, c), Fpb, Fgb, Cbar, th, &tau);
+    }
     {
       M3<T> Cbo = scaled(MD, I.pmass);
       for (int k = 0; k < 9; ++k) Cbo.a[k] += Cbar.a[k];
       st9(bar_in, n_slots, p, RC, Cbo);
     }
     {
-      // F_b = (I + dt Cn)^T Fn_b + stress path
       const M3<T> Cn = ld9(dyn_out, n_slots, p, RC);
       const M3<T> Fnb = ld9(bar_out, n_slots, p, RF);
       M3<T> IC = scaled(Cn, I.dt);
@@ -750,9 +788,8 @@ __global__ __launch_bounds__(256) void k_p2g_adj(const mpm_instance* inst, const
       for (int k = 0; k < 9; ++k) Fpb.a[k] += Fb.a[k];
       st9(bar_in, n_slots, p, RF, Fpb);
     }
-    const T detG = det(Fg);
     const M3<T> GiT = inv_t(Fg);
-    const T vol = skt * (-I.k4 * I.vol0) * detG;
+    const T vol = frob(skb, tau) * (-I.k4 * I.vol0) * detG;
     M3<T> Gb = ld9(Fg_bar, n_slots, p, 0);
     for (int k = 0; k < 9; ++k) Gb.a[k] += a_t * (Fgb.a[k] + vol * GiT.a[k]);
     st9(Fg_bar, n_slots, p, 0, Gb);
@@ -831,7 +868,7 @@ inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
 struct Workspace {
   int32_t *key, *slot, *count, *bsum;
-  void *payload, *partial, *gadj;
+  void *payload, *partial, *gadj, *md;
 };
 
 int64_t scan_blocks(int64_t n_bins) {
@@ -853,6 +890,7 @@ int64_t carve(const mpm_batch* b, size_t tsz, char* base, Workspace* w) {
   ws.payload = take((int64_t)tsz * kRecord * b->n_slots);
   ws.partial = take((int64_t)tsz * 4 * tile_nodes(b) * b->n_tiles);
   ws.gadj = take((int64_t)tsz * 4 * b->n_nodes);
+  ws.md = take((int64_t)tsz * 9 * b->n_slots);
   if (w) *w = ws;
   return off;
 }
@@ -868,6 +906,7 @@ int validate(const mpm_batch* b) {
   if (b->n_bins <= 0 || b->n_tiles <= 0) return set_err(MPM_EINVAL, "empty bin or tile table");
   if (b->block != 64 && b->block != 128 && b->block != 256)
     return set_err(MPM_EINVAL, "block must be 64, 128 or 256 (got %d)", b->block);
+  if (b->n_inst > 0 && b->inst == nullptr) return set_err(MPM_EINVAL, "batch tables missing");
   if (tile_index(b) < 0)
     return set_err(MPM_ENOTSUP, "tile shape %dx%dx%d not built", b->tile[0], b->tile[1], b->tile[2]);
   return MPM_OK;
@@ -943,12 +982,12 @@ int sort_impl(const mpm_batch* b, const T* dyn, int32_t* sortbuf, const Workspac
 }
 
 template <typename T, int TX, int TY, int TZ>
-void launch_tile_fwd(const mpm_batch* b, const int32_t* start, const Workspace& w, T* grid,
-                     hipStream_t st) {
+void launch_tile_fwd(const mpm_batch* b, const int32_t* start, const Workspace& w,
+                     const T* payload, T* grid, hipStream_t st) {
   {
   TimedLaunch tl(MPM_K_P2G_FWD, st);
   k_p2g_tile<T, TX, TY, TZ, true><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
-      b->inst, b->tile_inst, start, (const T*)w.payload, (T*)w.partial);
+      b->inst, b->tile_inst, start, payload, (T*)w.partial);
   }
   TimedLaunch tl(MPM_K_GRID_FWD, st);
   k_grid_fwd<T, TX, TY, TZ><<<grid1(b->n_nodes, 256), 256, 0, st>>>(
@@ -969,13 +1008,13 @@ void launch_tile_adj(const mpm_batch* b, const int32_t* start, const Workspace& 
 }
 
 template <typename T>
-void dispatch_fwd(int ti, const mpm_batch* b, const int32_t* start, const Workspace& w, T* grid,
-                  hipStream_t st) {
+void dispatch_fwd(int ti, const mpm_batch* b, const int32_t* start, const Workspace& w,
+                  const T* payload, T* grid, hipStream_t st) {
   switch (ti) {
-    case 0: launch_tile_fwd<T, 4, 8, 8>(b, start, w, grid, st); break;
-    case 1: launch_tile_fwd<T, 4, 4, 16>(b, start, w, grid, st); break;
-    case 2: launch_tile_fwd<T, 8, 8, 4>(b, start, w, grid, st); break;
-    default: launch_tile_fwd<T, 4, 4, 8>(b, start, w, grid, st); break;
+    case 0: launch_tile_fwd<T, 4, 8, 8>(b, start, w, payload, grid, st); break;
+    case 1: launch_tile_fwd<T, 4, 4, 16>(b, start, w, payload, grid, st); break;
+    case 2: launch_tile_fwd<T, 8, 8, 4>(b, start, w, payload, grid, st); break;
+    default: launch_tile_fwd<T, 4, 4, 8>(b, start, w, payload, grid, st); break;
   }
 }
 
@@ -992,10 +1031,10 @@ void dispatch_adj(int ti, const mpm_batch* b, const int32_t* start, const Worksp
 
 template <typename T>
 int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sched, T* dyn_out,
-                  T* grid, int32_t* sortbuf, int advect, void* ws, void* stream) {
+                  T* grid, int32_t* sortbuf, T* payload, int advect, void* ws, void* stream) {
   int rc = validate(b);
   if (rc) return rc;
-  if (!dyn_in || !Fg || !dyn_out || !grid || !sortbuf || !ws)
+  if (!dyn_in || !Fg || !dyn_out || !grid || !sortbuf || !payload || !ws)
     return set_err(MPM_EINVAL, "step_fwd: NULL buffer");
   if (dyn_in == dyn_out) return set_err(MPM_EINVAL, "step_fwd: dyn_out must not alias dyn_in");
   hipStream_t st = (hipStream_t)stream;
@@ -1010,10 +1049,10 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   TimedLaunch tl(MPM_K_PREP_FWD, st);
   with_model(b->model_mask, [&](auto M) {
     k_prep_fwd<T, decltype(M)::value><<<grid1(b->n_slots, bs), bs, 0, st>>>(
-        b->inst, b->slot_inst, b->n_slots, dyn_in, Fg, sched, rank, (T*)w.payload);
+        b->inst, b->slot_inst, b->n_slots, dyn_in, Fg, sched, rank, payload);
   });
   }
-  dispatch_fwd<T>(tile_index(b), b, start, w, grid, st);
+  dispatch_fwd<T>(tile_index(b), b, start, w, payload, grid, st);
   TimedLaunch tl(MPM_K_G2P, st);
   k_g2p<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in, grid,
                                                  dyn_out, advect);
@@ -1022,11 +1061,13 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
 
 template <typename T>
 int step_bwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sched,
-                  const T* dyn_out, const T* grid, const int32_t* sortbuf, const T* bar_out,
-                  T* bar_in, T* Fg_bar, double* theta_acc, int advect, void* ws, void* stream) {
+                  const T* dyn_out, const T* grid, const int32_t* sortbuf, const T* payload,
+                  const T* bar_out, T* bar_in, T* Fg_bar, double* theta_acc, int advect, void* ws,
+                  void* stream) {
   int rc = validate(b);
   if (rc) return rc;
-  if (!dyn_in || !Fg || !dyn_out || !grid || !sortbuf || !bar_out || !bar_in || !Fg_bar || !ws)
+  if (!dyn_in || !Fg || !dyn_out || !grid || !sortbuf || !payload || !bar_out || !bar_in ||
+      !Fg_bar || !ws)
     return set_err(MPM_EINVAL, "step_bwd: NULL buffer");
   if (bar_in == bar_out) return set_err(MPM_EINVAL, "step_bwd: bar_in must not alias bar_out");
   hipStream_t st = (hipStream_t)stream;
@@ -1041,11 +1082,18 @@ int step_bwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
                                                       bar_out, rank, (T*)w.payload, advect);
   }
   dispatch_adj<T>(tile_index(b), b, start, w, grid, advect, st);
+  const int32_t* sorted = sortbuf + b->n_slots;
+  {
   TimedLaunch tl(MPM_K_P2G_ADJ, st);
+  k_adj_gather<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(
+      b->inst, b->slot_inst, b->n_slots, sorted, payload, (const T*)w.payload, grid,
+      (const T*)w.gadj, bar_out, bar_in, (T*)w.md, advect);
+  }
+  TimedLaunch tl(MPM_K_ADJ_VJP, st);
   with_model(b->model_mask, [&](auto M) {
-    k_p2g_adj<T, decltype(M)::value><<<grid1(b->n_slots, 256), 256, 0, st>>>(
-        b->inst, b->slot_inst, b->n_slots, dyn_in, Fg, sched, dyn_out, grid, (const T*)w.gadj,
-        bar_out, bar_in, Fg_bar, theta_acc, advect);
+    k_adj_vjp<T, decltype(M)::value><<<grid1(b->n_slots, 256), 256, 0, st>>>(
+        b->inst, b->slot_inst, b->n_slots, rank, dyn_in, Fg, sched, dyn_out, (const T*)w.md,
+        bar_out, bar_in, Fg_bar, theta_acc);
   });
   return check_launch("step_bwd");
 }
@@ -1167,29 +1215,31 @@ int32_t mpm_sort_by_cell_f64(const mpm_batch* b, const double* dyn, int32_t* sor
 
 int32_t mpm_step_fwd_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
                          const float* sched, float* dyn_out, float* grid, int32_t* sortbuf,
-                         int32_t advect, void* ws, void* stream) {
-  return step_fwd_impl<float>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, advect, ws, stream);
+                         float* payload, int32_t advect, void* ws, void* stream) {
+  return step_fwd_impl<float>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, payload, advect, ws,
+                              stream);
 }
 int32_t mpm_step_fwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
                          const double* sched, double* dyn_out, double* grid, int32_t* sortbuf,
-                         int32_t advect, void* ws, void* stream) {
-  return step_fwd_impl<double>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, advect, ws, stream);
+                         double* payload, int32_t advect, void* ws, void* stream) {
+  return step_fwd_impl<double>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, payload, advect, ws,
+                               stream);
 }
 int32_t mpm_step_bwd_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
                          const float* sched, const float* dyn_out, const float* grid,
-                         const int32_t* sortbuf, const float* bar_out, float* bar_in,
-                         float* Fg_bar, double* theta_acc, int32_t advect, void* ws,
+                         const int32_t* sortbuf, const float* payload, const float* bar_out,
+                         float* bar_in, float* Fg_bar, double* theta_acc, int32_t advect, void* ws,
                          void* stream) {
-  return step_bwd_impl<float>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, bar_out, bar_in,
-                              Fg_bar, theta_acc, advect, ws, stream);
+  return step_bwd_impl<float>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, payload, bar_out,
+                              bar_in, Fg_bar, theta_acc, advect, ws, stream);
 }
 int32_t mpm_step_bwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
                          const double* sched, const double* dyn_out, const double* grid,
-                         const int32_t* sortbuf, const double* bar_out, double* bar_in,
-                         double* Fg_bar, double* theta_acc, int32_t advect, void* ws,
-                         void* stream) {
-  return step_bwd_impl<double>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, bar_out, bar_in,
-                               Fg_bar, theta_acc, advect, ws, stream);
+                         const int32_t* sortbuf, const double* payload, const double* bar_out,
+                         double* bar_in, double* Fg_bar, double* theta_acc, int32_t advect,
+                         void* ws, void* stream) {
+  return step_bwd_impl<double>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, payload, bar_out,
+                               bar_in, Fg_bar, theta_acc, advect, ws, stream);
 }
 
 }  // extern "C"

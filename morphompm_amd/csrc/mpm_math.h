@@ -209,9 +209,19 @@ MPM_HD M3<T> hencky_tau_core(const M3<T>& Fe, T mu, T lam) {
 // Fe cotangent from a precomputed eigen-decomposition b = U diag(beta) U^T.
 template <typename T>
 MPM_HD M3<T> hencky_fe_bar_eig(const M3<T>& Fe, const M3<T>& tb, T mu, T lam, const T beta[3],
-                               const M3<T>& U) {
+                               const M3<T>& U, M3<T>* tau_out = nullptr) {
   T lb[3];
   for (int i = 0; i < 3; ++i) lb[i] = std::log(beta[i]);
+  if (tau_out) {  // tau = U diag(mu ln b + lam/2 sum ln b) U^T from the same eigenpairs
+    const T S = T(0.5) * lam * (lb[0] + lb[1] + lb[2]);
+    const T h[3] = {mu * lb[0] + S, mu * lb[1] + S, mu * lb[2] + S};
+    for (int r = 0; r < 3; ++r)
+      for (int c = r; c < 3; ++c) {
+        const T v = U(r, 0) * h[0] * U(c, 0) + U(r, 1) * h[1] * U(c, 1) + U(r, 2) * h[2] * U(c, 2);
+        (*tau_out)(r, c) = v;
+        (*tau_out)(c, r) = v;
+      }
+  }
   M3<T> Mm = scaled(tb, mu);
   const T hl = T(0.5) * lam * trace(tb);
   Mm.a[0] += hl; Mm.a[4] += hl; Mm.a[8] += hl;
@@ -242,13 +252,15 @@ MPM_HD bool near_repeated(const float beta[3]) {
   return g < kNearRepeated * s;
 }
 
-MPM_HD M3<double> hencky_fe_bar(const M3<double>& Fe, const M3<double>& tb, double mu, double lam) {
+MPM_HD M3<double> hencky_fe_bar(const M3<double>& Fe, const M3<double>& tb, double mu, double lam,
+                                M3<double>* tau_out = nullptr) {
   double beta[3];
   M3<double> U;
   sym_eig3(mul_nt(Fe, Fe), beta, U);
-  return hencky_fe_bar_eig(Fe, tb, mu, lam, beta, U);
+  return hencky_fe_bar_eig(Fe, tb, mu, lam, beta, U, tau_out);
 }
-MPM_HD M3<float> hencky_fe_bar(const M3<float>& Fe, const M3<float>& tb, float mu, float lam) {
+MPM_HD M3<float> hencky_fe_bar(const M3<float>& Fe, const M3<float>& tb, float mu, float lam,
+                               M3<float>* tau_out = nullptr) {
   float beta[3];
   M3<float> U;
   sym_eig3(mul_nt(Fe, Fe), beta, U);
@@ -256,12 +268,15 @@ MPM_HD M3<float> hencky_fe_bar(const M3<float>& Fe, const M3<float>& tb, float m
   if (near_repeated(beta)) {
     const M3<double> Fed = cast3<double>(Fe);
     double bd[3];
-    M3<double> Ud;
+    M3<double> Ud, td;
     sym_eig3(mul_nt(Fed, Fed), bd, Ud);
-    return cast3<float>(hencky_fe_bar_eig(Fed, cast3<double>(tb), (double)mu, (double)lam, bd, Ud));
+    const M3<double> r = hencky_fe_bar_eig(Fed, cast3<double>(tb), (double)mu, (double)lam, bd, Ud,
+                                           tau_out ? &td : nullptr);
+    if (tau_out) *tau_out = cast3<float>(td);
+    return cast3<float>(r);
   }
 #endif
-  return hencky_fe_bar_eig(Fe, tb, mu, lam, beta, U);
+  return hencky_fe_bar_eig(Fe, tb, mu, lam, beta, U, tau_out);
 }
 
 MPM_HD M3<double> hencky_tau(const M3<double>& Fe, double mu, double lam) {
@@ -315,10 +330,11 @@ MPM_HD float hb_dnum(float u) {
 // VJP of tau w.r.t. (F, Fg, C) given tau_bar; theta_bar[0..1] get the HB
 // (tau_y, K) cotangents (zero for the elastic models).
 // NH: constitutive.py:31-39.  Hencky: constitutive.py:50-75.  HB: :134-147.
+// tau_out (optional) receives tau itself, from the same intermediates.
 template <typename T>
 MPM_HD void kirchhoff_vjp(const ModelParams<T>& mp, const M3<T>& F, const M3<T>& Fg, const M3<T>& C,
                           const M3<T>& tau_bar, M3<T>& F_bar, M3<T>& Fg_bar, M3<T>& C_bar,
-                          T theta_bar[2]) {
+                          T theta_bar[2], M3<T>* tau_out = nullptr) {
   const M3<T> tb = sym(tau_bar);
   const T trtb = trace(tb);
   theta_bar[0] = T(0);
@@ -338,6 +354,11 @@ MPM_HD void kirchhoff_vjp(const ModelParams<T>& mp, const M3<T>& F, const M3<T>&
     for (int i = 0; i < 9; ++i) C_bar.a[i] = T(2) * eta * tb.a[i] + cD * D.a[i];
     theta_bar[0] = T(2) * one_m_e / gd * tD;
     theta_bar[1] = T(2) * gpow * tD;
+    if (tau_out) {
+      const T vol = kappa * std::log(det(F) / det(Fg));
+      *tau_out = scaled(D, T(2) * eta);
+      tau_out->a[0] += vol; tau_out->a[4] += vol; tau_out->a[8] += vol;
+    }
     return;
   }
   C_bar = zero3<T>();
@@ -349,8 +370,14 @@ MPM_HD void kirchhoff_vjp(const ModelParams<T>& mp, const M3<T>& F, const M3<T>&
     const M3<T> tbFe = mul(tb, Fe);
     const M3<T> FeiT = inv_t(Fe);
     for (int i = 0; i < 9; ++i) Fe_bar.a[i] = T(2) * mu * tbFe.a[i] + lam * trtb * FeiT.a[i];
+    if (tau_out) {
+      const M3<T> b = mul_nt(Fe, Fe);
+      const T lj = lam * std::log(det(Fe));
+      for (int i = 0; i < 9; ++i) tau_out->a[i] = mu * b.a[i];
+      tau_out->a[0] += lj - mu; tau_out->a[4] += lj - mu; tau_out->a[8] += lj - mu;
+    }
   } else {
-    Fe_bar = hencky_fe_bar(Fe, tb, mu, lam);
+    Fe_bar = hencky_fe_bar(Fe, tb, mu, lam, tau_out);
   }
   F_bar = mul_nt(Fe_bar, Gi);                              // Fe_bar Gi^T
   // Fg_bar = -Gi^T F^T Fe_bar Gi^T

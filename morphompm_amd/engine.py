@@ -161,6 +161,9 @@ class BatchPlan:
     def new_sort(self):
         return torch.zeros(self.sortlen, dtype=torch.int32, device=self.device)
 
+    def new_payload(self, dtype):
+        return torch.zeros(self.n_slots, 16, dtype=dtype, device=self.device)
+
     # ------------------------------------------------------------ packing
     # ``perm`` (optional, per instance): slot j of instance i holds particle
     # perm[i][j] of the caller's arrays.  ``cell_order`` gives the order that
@@ -244,18 +247,19 @@ class BatchPlan:
         return cs[ends] - cs[starts]
 
     # ------------------------------------------------------------ kernels
-    def step_fwd(self, dyn_in, G, sched_row, dyn_out, grid, sortbuf, advect, stream=None):
+    def step_fwd(self, dyn_in, G, sched_row, dyn_out, grid, sortbuf, payload, advect,
+                 stream=None):
         f = getattr(_lib.lib(), f"mpm_step_fwd_{suffix(dyn_in.dtype)}")
         rc = f(ctypes.byref(self.c), ptr(dyn_in), ptr(G), sched_row, ptr(dyn_out), ptr(grid),
-               ptr(sortbuf), int(bool(advect)), ptr(self.workspace(dyn_in.dtype)),
+               ptr(sortbuf), ptr(payload), int(bool(advect)), ptr(self.workspace(dyn_in.dtype)),
                stream_ptr(stream))
         check(rc, "step_fwd")
 
-    def step_bwd(self, dyn_in, G, sched_row, dyn_out, grid, sortbuf, bar_out, bar_in, G_bar,
-                 theta_acc, advect, stream=None):
+    def step_bwd(self, dyn_in, G, sched_row, dyn_out, grid, sortbuf, payload, bar_out, bar_in,
+                 G_bar, theta_acc, advect, stream=None):
         f = getattr(_lib.lib(), f"mpm_step_bwd_{suffix(dyn_in.dtype)}")
         rc = f(ctypes.byref(self.c), ptr(dyn_in), ptr(G), sched_row, ptr(dyn_out), ptr(grid),
-               ptr(sortbuf), ptr(bar_out), ptr(bar_in), ptr(G_bar), ptr(theta_acc),
+               ptr(sortbuf), ptr(payload), ptr(bar_out), ptr(bar_in), ptr(G_bar), ptr(theta_acc),
                int(bool(advect)), ptr(self.workspace(dyn_in.dtype)), stream_ptr(stream))
         check(rc, "step_bwd")
 
@@ -313,6 +317,7 @@ class Trajectory:
         self.seg_dyn = torch.zeros(Ks, N_DYN, p.n_slots, dtype=dtype, device=p.device)
         self.seg_grid = torch.zeros(Ks, p.n_nodes, 4, dtype=dtype, device=p.device)
         self.seg_sort = torch.zeros(Ks, p.sortlen, dtype=torch.int32, device=p.device)
+        self.seg_payload = torch.zeros(Ks, p.n_slots, 16, dtype=dtype, device=p.device)
         self.G = p.new_mat(dtype)
         self.bar_a = p.new_dyn(dtype)
         self.bar_b = p.new_dyn(dtype)
@@ -339,7 +344,7 @@ class Trajectory:
         for k in range(L):
             dst = self.seg_dyn[k] if (keep or k < L - 1) else self.ckpt[s + 1]
             self.plan.step_fwd(self._src(s, k), self.G, self._a(t0 + k), dst, self.seg_grid[k],
-                               self.seg_sort[k], self.advect, self.stream)
+                               self.seg_sort[k], self.seg_payload[k], self.advect, self.stream)
         self._loaded = s if keep else -1
 
     def forward(self, dyn0, G=None):
@@ -385,7 +390,8 @@ class Trajectory:
             t0, L = self._seg(s)
             for k in range(L - 1, -1, -1):
                 self.plan.step_bwd(self._src(s, k), self.G, self._a(t0 + k), self.seg_dyn[k],
-                                   self.seg_grid[k], self.seg_sort[k], bar_out, bar_in,
+                                   self.seg_grid[k], self.seg_sort[k], self.seg_payload[k],
+                                   bar_out, bar_in,
                                    self.G_bar, th, self.advect, self.stream)
                 bar_out, bar_in = bar_in, bar_out
         theta = self.plan.theta_per_instance(self.theta_acc) if want_theta else None

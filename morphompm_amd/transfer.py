@@ -42,8 +42,8 @@ def plan_for(n, cfg, model, tile=None, block=256):
 
 
 class StepCache:
-    __slots__ = ("plan", "dtype", "dyn_in", "G", "dyn_out", "grid", "sort", "like", "advect",
-                 "cfg", "model")
+    __slots__ = ("plan", "dtype", "dyn_in", "G", "dyn_out", "grid", "sort", "payload", "like",
+                 "advect", "cfg", "model")
 
 
 def _pack_state(plan, state, dtype):
@@ -60,12 +60,14 @@ def step(state, model, cfg, advect=True):
     dyn_out = plan.new_dyn(dt)
     grid = plan.new_grid(dt)
     sort = plan.new_sort()
-    plan.step_fwd(dyn_in, G, None, dyn_out, grid, sort, advect)
+    payload = plan.new_payload(dt)
+    plan.step_fwd(dyn_in, G, None, dyn_out, grid, sort, payload, advect)
     x, v, C, F = plan.unpack_dyn(dyn_out)
     new = ParticleState(like.out(x), like.out(v), like.out(C), like.out(F),
                         like.out(plan.unpack_mat(G)))
     c = StepCache()
     c.plan, c.dtype, c.dyn_in, c.G, c.dyn_out, c.grid, c.sort = plan, dt, dyn_in, G, dyn_out, grid, sort
+    c.payload = payload
     c.like, c.advect, c.cfg, c.model = like, bool(advect), cfg, model
     return new, c
 
@@ -77,8 +79,8 @@ def step_vjp(cache, xn_b, vn_b, Cn_b, Fn_b, Fgn_b, advect=True):
     bar_out = p.pack_dyn([(xn_b, vn_b, Cn_b, Fn_b)], dt)
     G_bar = p.pack_mat([Fgn_b], dt)
     bar_in = p.new_dyn(dt)
-    p.step_bwd(cache.dyn_in, cache.G, None, cache.dyn_out, cache.grid, cache.sort, bar_out,
-               bar_in, G_bar, None, advect)
+    p.step_bwd(cache.dyn_in, cache.G, None, cache.dyn_out, cache.grid, cache.sort, cache.payload,
+               bar_out, bar_in, G_bar, None, advect)
     x, v, C, F = p.unpack_dyn(bar_in)
     o = cache.like.out
     return o(x), o(v), o(C), o(F), o(p.unpack_mat(G_bar))
