@@ -69,6 +69,9 @@ typedef struct mpm_instance {
   int32_t shift[3];   /* tile-grid offset in bins, 0 <= shift < tile: bin b lies in
                          tile (b + shift) / tile; aligns tiles with occupied cells */
   int32_t pad;
+  int32_t origin[3];  /* positions are stored relative to node origin*dx (whole cells),
+                         so FP32 keeps more bits of small displacements; node = base + origin */
+  int32_t pad2;
   double dx, dt, damping, pmass, vol0;
   double prm[6]; /* NH/Hencky: mu, lambda; HB: kappa, K, n, tau_y, m, eps */
 } mpm_instance;

@@ -25,7 +25,7 @@ class MpmInstance(ctypes.Structure):
     _fields_ = [("p_off", _I64), ("g_off", _I64), ("c_off", _I64), ("t_off", _I64),
                 ("n", _I32), ("nx", _I32), ("ny", _I32), ("nz", _I32),
                 ("tx", _I32), ("ty", _I32), ("tz", _I32), ("model", _I32),
-                ("shift", _I32 * 3), ("pad", _I32),
+                ("shift", _I32 * 3), ("pad", _I32), ("origin", _I32 * 3), ("pad2", _I32),
                 ("dx", ctypes.c_double), ("dt", ctypes.c_double),
                 ("damping", ctypes.c_double), ("pmass", ctypes.c_double),
                 ("vol0", ctypes.c_double), ("prm", ctypes.c_double * 6)]

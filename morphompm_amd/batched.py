@@ -49,7 +49,7 @@ class BatchedInverse:
             raise ValueError("all instances of a batch share the advect flag")
         self.advect = adv.pop()
         self.dtype = dtype
-        specs = [InstanceSpec.from_cfg(s.n, s.cfg, s.model, s.pts) for s in scenes]
+        specs = [InstanceSpec.from_cfg(s.n, s.cfg, s.model, s.pts, center=True) for s in scenes]
         self.plan = p = BatchPlan(specs, tile=tile, block=block)
         K = checkpoint_every or scenes[0].checkpoint_every
         sched = np.stack([s.sched if s.sched is not None else np.ones(self.n_steps)

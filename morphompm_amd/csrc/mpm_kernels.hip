@@ -60,7 +60,7 @@ int check_launch(const char* what) {
 template <typename T>
 struct Inst {
   int64_t p_off, g_off, c_off, t_off;
-  int n, nx, ny, nz, tx, ty, tz, sx, sy, sz;
+  int n, nx, ny, nz, tx, ty, tz, sx, sy, sz, ox, oy, oz;
   T dx, inv_dx, dt, s, pmass, vol0, k4;
   ModelParams<T> mp;
 };
@@ -74,6 +74,7 @@ __device__ __forceinline__ Inst<T> load_inst(const mpm_instance* inst, int i) {
   r.p_off = I.p_off; r.g_off = I.g_off; r.c_off = I.c_off; r.t_off = I.t_off;
   r.n = I.n; r.nx = I.nx; r.ny = I.ny; r.nz = I.nz; r.tx = I.tx; r.ty = I.ty; r.tz = I.tz;
   r.sx = I.shift[0]; r.sy = I.shift[1]; r.sz = I.shift[2];
+  r.ox = I.origin[0]; r.oy = I.origin[1]; r.oz = I.origin[2];
   r.dx = (T)I.dx;
   r.inv_dx = (T)(1.0 / I.dx);
   r.dt = (T)I.dt;
@@ -150,9 +151,17 @@ __device__ __forceinline__ M3<T> grown(const M3<T>& G, T a) {
   return Fg;
 }
 
+// Stencil base node (in grid nodes) of a position stored relative to the
+// instance origin: base = floor(x/dx - 1/2) + origin.
 template <typename T>
-__device__ __forceinline__ void stencil_base(const T x[3], T inv_dx, int base[3], T fx[3]) {
-  for (int a = 0; a < 3; ++a) base[a] = cell_base(x[a], inv_dx, fx[a]);
+__device__ __forceinline__ void stencil_base(const T x[3], T inv_dx, const int origin[3],
+                                             int base[3], T fx[3]) {
+  for (int a = 0; a < 3; ++a) base[a] = cell_base(x[a], inv_dx, fx[a]) + origin[a];
+}
+template <typename T>
+__device__ __forceinline__ void stencil_base(const T x[3], const Inst<T>& I, int base[3], T fx[3]) {
+  const int o[3] = {I.ox, I.oy, I.oz};
+  stencil_base(x, I.inv_dx, o, base, fx);
 }
 
 // Instance of this block: blocks never straddle a 256-slot chunk (block sizes
@@ -213,7 +222,7 @@ __global__ __launch_bounds__(256) void k_sort_key(const mpm_instance* inst, cons
       T x[3], fx[3];
       int base[3];
       ld3(dyn, n_slots, p, RX, x);
-      stencil_base(x, (T)(1.0 / I.dx), base, fx);
+      stencil_base(x, (T)(1.0 / I.dx), I.origin, base, fx);
       const int bx = base[0] + 2, by = base[1] + 2, bz = base[2] + 2;
       if (bx >= 0 && bx < I.nx + 2 && by >= 0 && by < I.ny + 2 && bz >= 0 && bz < I.nz + 2)
         k = bin_key(I, bx, by, bz, TX, TY, TZ);
@@ -372,7 +381,7 @@ __global__ __launch_bounds__(256) void k_prep_fwd(const mpm_instance* inst, cons
   const M3<T> C = ld9(dyn, n_slots, p, RC);
   const M3<T> F = ld9(dyn, n_slots, p, RF);
   const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), sched ? sched[ii] : T(1));
-  stencil_base(x, I.inv_dx, base, fx);
+  stencil_base(x, I, base, fx);
   const M3<T> tau = kirchhoff(I.mp, F, Fg, C);
   const T c = -I.k4 * I.vol0 * det(Fg);
   M3<T> Ap;
@@ -527,7 +536,7 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
   T x[3], fx[3];
   int base[3];
   ld3(dyn_in, n_slots, p, RX, x);
-  stencil_base(x, I.inv_dx, base, fx);
+  stencil_base(x, I, base, fx);
   const AxisNodes nx = axis_nodes(base[0], I.nx), ny = axis_nodes(base[1], I.ny),
                   nz = axis_nodes(base[2], I.nz);
   T w[3][3];
@@ -581,7 +590,7 @@ __global__ __launch_bounds__(256) void k_prep_adj(const mpm_instance* inst, cons
   T x[3], fx[3], nvb[3];
   int base[3];
   ld3(dyn_in, n_slots, p, RX, x);
-  stencil_base(x, I.inv_dx, base, fx);
+  stencil_base(x, I, base, fx);
   ld3(bar_out, n_slots, p, RV, nvb);
   if (advect) {
     T xb[3];

@@ -53,21 +53,26 @@ class InstanceSpec:
     kind: int
     prm: tuple
     bin_lo: tuple = None   # lowest occupied bin per axis at the start (tile alignment)
+    origin: tuple = (0, 0, 0)   # positions stored relative to origin * dx (whole cells)
 
     @staticmethod
-    def from_cfg(n, cfg, model, pts=None):
+    def from_cfg(n, cfg, model, pts=None, center=False):
         """``pts`` (optional rest positions) aligns the P2G tiles with the lowest
-        occupied bin of each axis."""
+        occupied bin of each axis; ``center`` also stores positions relative to
+        the whole cell nearest their mean (more FP32 bits for displacements)."""
         kind, prm = model_spec(model)
-        lo = None
+        lo, origin = None, (0, 0, 0)
         if pts is not None:
             x = np.asarray(pts.detach().cpu() if isinstance(pts, torch.Tensor) else pts,
                            dtype=np.float64)
             if len(x):
                 lo = tuple(int(v) for v in
                            np.maximum(np.floor(x * (1.0 / cfg.dx) - 0.5).min(0) + 2, 0))
+                if center:
+                    origin = tuple(int(v) for v in np.rint(x.mean(0) / cfg.dx))
         return InstanceSpec(int(n), tuple(cfg.shape), float(cfg.dx), float(cfg.dt),
-                            float(cfg.damping), float(cfg.pmass), float(cfg.vol0), kind, prm, lo)
+                            float(cfg.damping), float(cfg.pmass), float(cfg.vol0), kind, prm, lo,
+                            origin)
 
 
 def _ceil(a, b):
@@ -94,6 +99,7 @@ class BatchPlan:
                           _ceil(nz + 2 + sh[2], self.tile[2]))
             I = insts[i]
             I.shift[0], I.shift[1], I.shift[2] = sh
+            I.origin[0], I.origin[1], I.origin[2] = (int(o) for o in s.origin)
             I.p_off, I.g_off, I.c_off, I.t_off = p_off, g_off, c_off, t_off
             I.n, I.nx, I.ny, I.nz, I.tx, I.ty, I.tz = s.n, nx, ny, nz, tx, ty, tz
             I.model = s.kind
@@ -139,6 +145,14 @@ class BatchPlan:
         for i, s in enumerate(self.specs):
             live[self.p_off[i]:self.p_off[i] + s.n] = True
         self.live = torch.from_numpy(live).to(self.device)
+
+    def set_prm(self, i, prm6):
+        """Replace instance i's model parameters (e.g. Herschel-Bulkley tau_y, K)
+        in the device instance table, stream-ordered on the current stream."""
+        for k in range(6):
+            self._insts[i].prm[k] = float(prm6[k])
+        raw = torch.frombuffer(bytearray(bytes(self._insts)), dtype=torch.uint8)
+        self.d_inst.copy_(raw.to(self.device, non_blocking=False))
 
     # ------------------------------------------------------------ buffers
     def workspace(self, dtype):
@@ -198,12 +212,20 @@ class BatchPlan:
             t = t[torch.as_tensor(perm[i], device=self.device)]
         return t.T
 
-    def pack_dyn(self, parts, dtype, perm=None):
-        """parts: list over instances of (x, v, C, F) with shapes (n,3),(n,3),(n,3,3),(n,3,3)."""
+    def origin_offset(self, i, dtype=torch.float64):
+        s = self.specs[i]
+        return torch.tensor([o * s.dx for o in s.origin], dtype=dtype, device=self.device)
+
+    def pack_dyn(self, parts, dtype, perm=None, positions=True):
+        """parts: list over instances of (x, v, C, F) with shapes (n,3),(n,3),(n,3,3),(n,3,3).
+        ``positions``: x is a position (stored relative to the instance origin);
+        False for cotangents."""
         dyn = self.new_dyn(dtype)
         for i, (x, v, C, F) in enumerate(parts):
             s = self.sl(i)
             n = self.specs[i].n
+            if positions and any(self.specs[i].origin):
+                x = _dev(x, torch.float64, self.device).reshape(n, 3) - self.origin_offset(i)
             dyn[ROW_X:ROW_X + 3, s] = self._cols(x, i, n, 3, dtype, perm)
             dyn[ROW_V:ROW_V + 3, s] = self._cols(v, i, n, 3, dtype, perm)
             dyn[ROW_C:ROW_C + 9, s] = self._cols(C, i, n, 9, dtype, perm)
@@ -224,11 +246,14 @@ class BatchPlan:
                                                                           device=self.device)
         return t[inv]
 
-    def unpack_dyn(self, dyn, i=0, perm=None):
+    def unpack_dyn(self, dyn, i=0, perm=None, positions=True):
         s = self.sl(i)
         n = self.specs[i].n
         u = lambda t: self._uncols(t, i, perm)
-        return (u(dyn[ROW_X:ROW_X + 3, s].T), u(dyn[ROW_V:ROW_V + 3, s].T),
+        x = u(dyn[ROW_X:ROW_X + 3, s].T)
+        if positions and any(self.specs[i].origin):
+            x = (x.to(torch.float64) + self.origin_offset(i)).to(dyn.dtype)
+        return (x, u(dyn[ROW_V:ROW_V + 3, s].T),
                 u(dyn[ROW_C:ROW_C + 9, s].T).reshape(n, 3, 3),
                 u(dyn[ROW_F:ROW_F + 9, s].T).reshape(n, 3, 3))
 

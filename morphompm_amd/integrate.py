@@ -52,10 +52,10 @@ def rollout_vjp(caches, xf_b, vf_b, Cf_b, Ff_b, Fgf_b, advect=False):
     if bool(advect) != h.traj.advect:
         raise ValueError("rollout_vjp: advect must match the rollout")
     p, dt = h.plan, h.traj.dtype
-    bar = p.pack_dyn([(xf_b, vf_b, Cf_b, Ff_b)], dt)
+    bar = p.pack_dyn([(xf_b, vf_b, Cf_b, Ff_b)], dt, positions=False)
     Gb = p.pack_mat([Fgf_b], dt)
     bar0, G_bar, _ = h.traj.backward(bar, Gb)
-    x, v, C, F = p.unpack_dyn(bar0)
+    x, v, C, F = p.unpack_dyn(bar0, positions=False)
     o = h.like.out
     return o(x), o(v), o(C), o(F), o(p.unpack_mat(G_bar))
 

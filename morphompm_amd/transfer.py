@@ -76,11 +76,11 @@ def step_vjp(cache, xn_b, vn_b, Cn_b, Fn_b, Fgn_b, advect=True):
     """Reverse of ``step``; ``advect`` must match the forward call
     (python/morphompm/transfer.py:83-86)."""
     p, dt = cache.plan, cache.dtype
-    bar_out = p.pack_dyn([(xn_b, vn_b, Cn_b, Fn_b)], dt)
+    bar_out = p.pack_dyn([(xn_b, vn_b, Cn_b, Fn_b)], dt, positions=False)
     G_bar = p.pack_mat([Fgn_b], dt)
     bar_in = p.new_dyn(dt)
     p.step_bwd(cache.dyn_in, cache.G, None, cache.dyn_out, cache.grid, cache.sort, cache.payload,
                bar_out, bar_in, G_bar, None, advect)
-    x, v, C, F = p.unpack_dyn(bar_in)
+    x, v, C, F = p.unpack_dyn(bar_in, positions=False)
     o = cache.like.out
     return o(x), o(v), o(C), o(F), o(p.unpack_mat(G_bar))

@@ -90,7 +90,7 @@ def test_batched_equals_single_instance_bitwise():
     sc = _small_batch(("nh", "hencky", "nh"), n_steps=8)
     bi, _, _ = _run_batch(sc, torch.float32)
     for i, s in enumerate(sc):
-        p = BatchPlan([InstanceSpec.from_cfg(s.n, s.cfg, s.model, s.pts)])
+        p = BatchPlan([InstanceSpec.from_cfg(s.n, s.cfg, s.model, s.pts, center=True)])
         perm = p.cell_order([s.pts])
         tr = Trajectory(p, torch.float32, s.n_steps, True, 5, s.sched)
         dyn0 = p.pack_dyn([(s.pts, np.zeros((s.n, 3)), np.zeros((s.n, 3, 3)),

