@@ -31,7 +31,8 @@ HBM_PEAK_GBS = 8000.0
 # kernel's compulsory bytes per launch (FP32): n particles, a active grid
 # nodes, nb bins.  Per particle: records of 4-byte words read + written.
 KERNELS = {
-    0: ("sort", lambda n, a, nb: 20 * n + 4 * nb),          # x in, key/rank/sorted; bin counts
+    0: ("sort", lambda n, a, nb: 32 * n + 12 * nb),         # x in twice, key/slot/rank/sorted;
+                                                             # counts + starts of occupied bins
     1: ("prep_fwd", lambda n, a, nb: 196 * n),              # x v C F G in, 16-word payload out
     2: ("p2g_fwd_tile", lambda n, a, nb: 64 * n + 16 * a),  # payload in, node partials out
     3: ("grid_fwd", lambda n, a, nb: 32 * a),               # partials in, (m, v) out
@@ -218,12 +219,15 @@ def main():
         ktab[kname] = {"id": kid, "avg_ms": float(np.mean(d)), "launches_timed": cnt}
     grids = bi.traj.seg_grid
     active = float(np.mean([int((grids[k][:, 0] > 1e-12).sum()) for k in range(grids.shape[0])]))
+    # occupied bins of this step's sort: n_active units (last word of a sort record)
+    occ_bins = float(np.mean([int(bi.traj.seg_sort[k][-1]) for k in range(bi.traj.seg_sort.shape[0])])
+                     ) * int(np.prod(bi.plan.tile))
     iter_launch = {k: (a.sim_steps if k in ADJ_KERNELS else
                        a.sim_steps + (a.sim_steps - bi.traj._seg(bi.traj.nseg - 1)[1]))
                    for k in KERNELS}
     for kname, e in ktab.items():
         kid = e["id"]
-        e["algorithmic_bytes_per_launch"] = KERNELS[kid][1](n_total, active, bi.plan.n_bins)
+        e["algorithmic_bytes_per_launch"] = KERNELS[kid][1](n_total, active, occ_bins)
         e["achieved_GBs"] = e["algorithmic_bytes_per_launch"] / (e["avg_ms"] / 1e3) / 1e9
         e["ms_per_iteration"] = e["avg_ms"] * iter_launch[kid]
     # the dominant single kernel (the sort entry is a 7-launch sequence)

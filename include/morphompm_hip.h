@@ -18,10 +18,13 @@
  *                       Fg = I + a (G - I) with a = sched[instance]
  *                       (sched == NULL means a = 1, i.e. Fg = G exactly)
  * Grid layout: grid [n_nodes][4]: per node (mass, vx, vy, vz).
- * Sort record for one step: sortbuf int32 [2 n_slots + n_bins + 1]:
- *   rank[n_slots] | sorted[n_slots] | bin_start[n_bins + 1]
- *   (each instance's sorted range is its own slot range; padding slots sort
- *   into the instance's dead bin)
+ * Sort record for one step: sortbuf int32 [2 n_slots + n_bins + 2 n_tiles + 2]:
+ *   rank[n_slots] | sorted[n_slots] | bin_start[n_bins + 1] |
+ *   unit_rank[n_tiles] | active_unit[n_tiles] | n_active[1]
+ *   Only the units (tiles) that hold particles this step get bins: unit u has
+ *   compact index unit_rank[u] (-1 when empty), active_unit inverts it, and
+ *   bin_start covers n_active * TB bins.  Each instance's sorted range is its
+ *   own slot range; padding slots sort into the instance's dead unit.
  * P2G payload for one step: payload [n_slots][16] in sorted order, per
  *   particle (fx[3], q[3], A'[9], cell) -- the forward step's momentum map
  *   m_i += w pmass, mom_i += w (q + A' o) for stencil offset o, and the packed
@@ -41,7 +44,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 1
+#define MPM_ABI_VERSION 2
 #define MPM_SLOT_ALIGN 256 /* particle and node ranges of an instance start at multiples of this */
 
 enum { MPM_OK = 0, MPM_EINVAL = -1, MPM_ELAUNCH = -2, MPM_ENOTSUP = -3 };
@@ -50,18 +53,17 @@ enum { MPM_NEO_HOOKEAN = 0, MPM_HENCKY = 1, MPM_HERSCHEL_BULKLEY = 2 };
 /* One independent simulation (python/morphompm/config.py:21-40 SimConfig plus
  * its constitutive model).  Node grid nx*ny*nz covers [0, n*dx) per axis.
  * Particles are binned by cell base+2 in [0, n+2) per axis; with b' = bin +
- * shift, the bins are grouped into tiles of tile[0]*tile[1]*tile[2] bins
- * (tx*ty*tz tiles, tx = ceil((nx+2+shift[0])/tile[0]) ...) and numbered
- * tile-major:
- *   key = c_off + ((itx*ty + ity)*tz + itz) * TB + local bin,  itx = b'x / tile[0],
- *   local bin from b' mod tile, TB = tile volume,
- * followed by one dead bin (key c_off + tx*ty*tz*TB) for particles with no
- * in-grid node.  An instance owns tx*ty*tz*TB + 1 bins. */
+ * shift, the bins are grouped into tiles ("units") of tile[0]*tile[1]*tile[2]
+ * bins (tx*ty*tz of them, tx = ceil((nx+2+shift[0])/tile[0]) ...), numbered
+ *   unit = t_off + (itx*ty + ity)*tz + itz,  itx = b'x / tile[0],
+ * with the bin's place in its unit from b' mod tile, followed by one dead unit
+ * (t_off + tx*ty*tz) that holds particles with no in-grid node.  An instance
+ * owns tx*ty*tz + 1 units. */
 typedef struct mpm_instance {
   int64_t p_off; /* first particle slot, multiple of MPM_SLOT_ALIGN */
   int64_t g_off; /* first node, multiple of MPM_SLOT_ALIGN */
-  int64_t c_off; /* first bin */
-  int64_t t_off; /* first P2G tile */
+  int64_t reserved;
+  int64_t t_off; /* first unit */
   int32_t n;     /* live particles */
   int32_t nx, ny, nz;
   int32_t tx, ty, tz; /* P2G tiles per axis */
@@ -83,22 +85,25 @@ typedef struct mpm_batch {
   int32_t tile[3];   /* P2G tile shape in bins (x, y, z) */
   int64_t n_slots;   /* particle slots incl. padding, multiple of MPM_SLOT_ALIGN */
   int64_t n_nodes;   /* node slots incl. padding, multiple of MPM_SLOT_ALIGN */
-  int64_t n_bins;    /* bins of all instances */
-  int64_t n_tiles;   /* P2G tiles of all instances */
+  int64_t n_bins;    /* bin capacity: n_tiles * tile volume */
+  int64_t n_tiles;   /* units of all instances (tiles + one dead unit each) */
   int32_t block;     /* threads per block of the particle kernels: 64, 128 or 256 */
   int32_t model_mask; /* bit m set when some instance uses model m; one bit set selects
                          kernels specialised for that model */
   const mpm_instance* inst;  /* device [n_inst] */
   const int32_t* slot_inst;  /* device [n_slots / 256]: instance owning each 256 slots */
   const int32_t* node_inst;  /* device [n_nodes / 256] */
-  const int32_t* tile_inst;  /* device [n_tiles] */
+  const int32_t* tile_inst;  /* device [n_tiles]: instance owning each unit */
 } mpm_batch;
 
 int32_t mpm_abi_version(void);
 const char* mpm_last_error(void);
 
 /* Bytes of scratch `ws` the step functions need for this batch and dtype
- * (0 = f32, 1 = f64).  Negative on error. */
+ * (0 = f32, 1 = f64).  Negative on error.  `ws` must be zero-filled once
+ * before its first use; every sort leaves its bin counters and unit flags
+ * cleared again, so a workspace is reused across steps (one stream at a time)
+ * without further clearing. */
 int64_t mpm_workspace_bytes(const mpm_batch* b, int32_t dtype);
 
 /* Tile shapes this build instantiates, as x,y,z triples; returns the count. */

@@ -11,7 +11,7 @@ import torch
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libmorphompm_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 SLOT_ALIGN = 256
 
 NEO_HOOKEAN, HENCKY, HERSCHEL_BULKLEY = 0, 1, 2
@@ -22,7 +22,7 @@ _I64 = ctypes.c_int64
 
 
 class MpmInstance(ctypes.Structure):
-    _fields_ = [("p_off", _I64), ("g_off", _I64), ("c_off", _I64), ("t_off", _I64),
+    _fields_ = [("p_off", _I64), ("g_off", _I64), ("reserved", _I64), ("t_off", _I64),
                 ("n", _I32), ("nx", _I32), ("ny", _I32), ("nz", _I32),
                 ("tx", _I32), ("ty", _I32), ("tz", _I32), ("model", _I32),
                 ("shift", _I32 * 3), ("pad", _I32), ("origin", _I32 * 3), ("pad2", _I32),

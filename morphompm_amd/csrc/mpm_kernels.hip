@@ -59,7 +59,7 @@ int check_launch(const char* what) {
 // Per-instance constants in working precision.
 template <typename T>
 struct Inst {
-  int64_t p_off, g_off, c_off, t_off;
+  int64_t p_off, g_off, t_off;
   int n, nx, ny, nz, tx, ty, tz, sx, sy, sz, ox, oy, oz;
   T dx, inv_dx, dt, s, pmass, vol0, k4;
   ModelParams<T> mp;
@@ -71,7 +71,7 @@ template <typename T, int MODEL = -1>
 __device__ __forceinline__ Inst<T> load_inst(const mpm_instance* inst, int i) {
   const mpm_instance& I = inst[i];
   Inst<T> r;
-  r.p_off = I.p_off; r.g_off = I.g_off; r.c_off = I.c_off; r.t_off = I.t_off;
+  r.p_off = I.p_off; r.g_off = I.g_off; r.t_off = I.t_off;
   r.n = I.n; r.nx = I.nx; r.ny = I.ny; r.nz = I.nz; r.tx = I.tx; r.ty = I.ty; r.tz = I.tz;
   r.sx = I.shift[0]; r.sy = I.shift[1]; r.sz = I.shift[2];
   r.ox = I.origin[0]; r.oy = I.origin[1]; r.oz = I.origin[2];
@@ -176,56 +176,151 @@ constexpr int RX = 0, RV = 3, RC = 6, RF = 15;
 
 // ---------------------------------------------------------------------------
 // sort
+//
+// Particles are binned by cell (base+2 per axis, (nx+2)(ny+2)(nz+2) cells per
+// instance).  Cells are grouped into P2G tiles of TB = TX*TY*TZ cells, and
+// every instance owns tx*ty*tz tile units followed by one dead unit (particles
+// with no in-grid node, then the padding slots).  Per step only the occupied
+// units get bins: unit_rank[u] is the compact index of an occupied unit (-1
+// otherwise), active_unit[a] its inverse and n_active their number; a
+// particle's key is unit_rank[unit] * TB + its cell's place in the tile.  Units
+// are ranked in order, so each instance's sorted range is its own slot range.
 // ---------------------------------------------------------------------------
-__global__ void k_zero_i32(int32_t* a, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = 0;
-}
 
-// Bins are numbered tile-major: the (nx+2)(ny+2)(nz+2) bins of an instance
-// are grouped into P2G tiles of TX*TY*TZ bins; key = c_off + tile * TB + local
-// bin, so each tile's particles are contiguous in sorted order.  The last key
-// of an instance (c_off + tiles * TB) is the dead bin for particles with no
-// in-grid node.
-__device__ __forceinline__ int64_t bin_key(const mpm_instance& I, int bx, int by, int bz, int TX,
-                                           int TY, int TZ) {
+// Unit (tile) of in-grid bin (bx, by, bz) and the bin's place in the tile.
+__device__ __forceinline__ void bin_unit(const mpm_instance& I, int bx, int by, int bz, int TX,
+                                         int TY, int TZ, int64_t& unit, int& local) {
   bx += I.shift[0];
   by += I.shift[1];
   bz += I.shift[2];
   const int itx = bx / TX, ity = by / TY, itz = bz / TZ;
-  const int64_t t = ((int64_t)itx * I.ty + ity) * I.tz + itz;
-  const int l = ((bx - itx * TX) * TY + (by - ity * TY)) * TZ + (bz - itz * TZ);
-  return I.c_off + t * (TX * TY * TZ) + l;
+  unit = I.t_off + ((int64_t)itx * I.ty + ity) * I.tz + itz;
+  local = ((bx - itx * TX) * TY + (by - ity * TY)) * TZ + (bz - itz * TZ);
+}
+
+// Unit and place of a live particle; false when no stencil node is in the grid.
+template <typename T>
+__device__ __forceinline__ bool particle_unit(const mpm_instance& I, const T* dyn, int64_t n_slots,
+                                              int64_t p, int TX, int TY, int TZ, int64_t& unit,
+                                              int& local) {
+  T x[3], fx[3];
+  int base[3];
+  ld3(dyn, n_slots, p, RX, x);
+  stencil_base(x, (T)(1.0 / I.dx), I.origin, base, fx);
+  const int bx = base[0] + 2, by = base[1] + 2, bz = base[2] + 2;
+  if (bx < 0 || bx >= I.nx + 2 || by < 0 || by >= I.ny + 2 || bz < 0 || bz >= I.nz + 2)
+    return false;
+  bin_unit(I, bx, by, bz, TX, TY, TZ, unit, local);
+  return true;
+}
+
+__device__ __forceinline__ int64_t dead_unit(const mpm_instance& I) {
+  return I.t_off + (int64_t)I.tx * I.ty * I.tz;
+}
+
+// Flags the units that hold particles (plain stores of 1: order-free).
+template <typename T>
+__global__ __launch_bounds__(256) void k_unit_mark(const mpm_instance* inst,
+                                                   const int32_t* slot_inst, int64_t n_slots,
+                                                   const T* dyn, int TX, int TY, int TZ,
+                                                   int32_t* unit_flag) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_slots) return;
+  const mpm_instance& I = inst[block_inst(slot_inst)];
+  if (p - I.p_off >= I.n) return;
+  int64_t unit;
+  int local;
+  if (particle_unit(I, dyn, n_slots, p, TX, TY, TZ, unit, local)) unit_flag[unit] = 1;
+}
+
+// Units are ranked in two passes over 256-unit blocks: k_unit_count counts
+// each block's live units (flagged, or the dead unit -- the last unit of its
+// instance), k_unit_rank adds the counts of the blocks before it, ranks the
+// live units with a ballot, writes unit_rank, active_unit and n_active, and
+// clears the flags for the next step.
+__device__ __forceinline__ int32_t unit_live(const int32_t* unit_inst, int64_t n_units,
+                                             const int32_t* unit_flag, int64_t u) {
+  if (u >= n_units) return 0;
+  const bool dead = (u + 1 == n_units) || (unit_inst[u + 1] != unit_inst[u]);
+  return (unit_flag[u] != 0 || dead) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_unit_count(const int32_t* unit_inst, int64_t n_units,
+                                                    const int32_t* unit_flag, int32_t* ucount) {
+  __shared__ int32_t wsum[4];
+  const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const unsigned long long m = __ballot(unit_live(unit_inst, n_units, unit_flag, u));
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) ucount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// Sum of v[0, end) by the 256 threads of a block (result valid in every thread).
+__device__ __forceinline__ int32_t block_prefix(const int32_t* v, int64_t end, int32_t* red) {
+  int32_t s = 0;
+  for (int64_t i = threadIdx.x; i < end; i += 256) s += v[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < (unsigned)off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  const int32_t r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_unit_rank(const int32_t* unit_inst, int64_t n_units,
+                                                   int32_t* unit_flag, const int32_t* ucount,
+                                                   int32_t* unit_rank, int32_t* active_unit,
+                                                   int32_t* n_active) {
+  __shared__ int32_t red[256];
+  __shared__ int32_t wsum[4];
+  const int32_t base = block_prefix(ucount, blockIdx.x, red);
+  const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int32_t f = unit_live(unit_inst, n_units, unit_flag, u);
+  const unsigned long long m = __ballot(f);
+  if (lane == 0) wsum[wave] = __popcll(m);
+  __syncthreads();
+  int32_t r = base + __popcll(m & ((1ull << lane) - 1));
+  for (int w = 0; w < wave; ++w) r += wsum[w];
+  if (u < n_units) {
+    unit_rank[u] = f ? r : -1;
+    if (f) active_unit[r] = (int32_t)u;
+    unit_flag[u] = 0;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+    *n_active = base + wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_sort_key(const mpm_instance* inst, const int32_t* slot_inst,
                                                   int64_t n_slots, const T* dyn, int TX, int TY,
-                                                  int TZ, int32_t* key, int32_t* slot,
-                                                  int32_t* count) {
+                                                  int TZ, const int32_t* unit_rank, int32_t* key,
+                                                  int32_t* slot, int32_t* count) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const mpm_instance& I = inst[block_inst(slot_inst)];
-  // Padding slots belong to the dead bin too, so each instance's sorted range
-  // is exactly its own slot range [p_off, p_off + padded count): they take the
-  // tail of that range in slot order (k_place) and no atomic slot; the first
-  // padding slot adds their count to the dead bin once.
+  const int TB = TX * TY * TZ;
+  // The dead unit's bin 0 holds live particles with no in-grid node and its
+  // bin 1 the padding slots, so each instance's sorted range is exactly its
+  // own slot range [p_off, p_off + padded count): padding slots take the tail
+  // of that range in slot order (k_place) and no atomic slot; the first
+  // padding slot sets bin 1's count once.
   int64_t k = -1;
   if (p < n_slots) {
-    const int64_t dead = I.c_off + (int64_t)I.tx * I.ty * I.tz * (TX * TY * TZ);
+    const int64_t dead = (int64_t)unit_rank[dead_unit(I)] * TB;
     const int64_t local = p - I.p_off;
     if (local == I.n) {
       const int64_t padded = ((int64_t)I.n + kChunk - 1) / kChunk * kChunk;
-      atomicAdd(&count[dead], (int32_t)((padded > 0 ? padded : kChunk) - I.n));
+      atomicAdd(&count[dead + 1], (int32_t)((padded > 0 ? padded : kChunk) - I.n));
     }
     if (local < I.n) {
-      k = dead;
-      T x[3], fx[3];
-      int base[3];
-      ld3(dyn, n_slots, p, RX, x);
-      stencil_base(x, (T)(1.0 / I.dx), I.origin, base, fx);
-      const int bx = base[0] + 2, by = base[1] + 2, bz = base[2] + 2;
-      if (bx >= 0 && bx < I.nx + 2 && by >= 0 && by < I.ny + 2 && bz >= 0 && bz < I.nz + 2)
-        k = bin_key(I, bx, by, bz, TX, TY, TZ);
+      int64_t unit;
+      int l;
+      k = particle_unit(I, dyn, n_slots, p, TX, TY, TZ, unit, l)
+              ? (int64_t)unit_rank[unit] * TB + l
+              : dead;
     }
   }
   // One atomic per run of equal keys in adjacent lanes (particles arrive
@@ -248,14 +343,22 @@ __global__ __launch_bounds__(256) void k_sort_key(const mpm_instance* inst, cons
   }
 }
 
-// Exclusive scan of count[n] -> start[n + 1], in three passes.
-__global__ void k_scan_blocks(const int32_t* count, int32_t* start, int32_t* bsum, int64_t n) {
+// Exclusive scan of count[n] -> start[n + 1] over the n = n_active * TB bins
+// of this step, in three passes; the counts are cleared for the next step.
+__global__ void k_scan_blocks(int32_t* count, int32_t* start, int32_t* bsum,
+                              const int32_t* n_active, int TB) {
   __shared__ int32_t sh[kScanBlock];
+  const int64_t n = (int64_t)__builtin_amdgcn_readfirstlane(*n_active) * TB;
   const int64_t i0 = ((int64_t)blockIdx.x * kScanBlock + threadIdx.x) * kScanPer;
+  if ((int64_t)blockIdx.x * kScanBlock * kScanPer >= n) return;
   int32_t v[kScanPer];
   int32_t tot = 0;
   for (int k = 0; k < kScanPer; ++k) {
-    v[k] = (i0 + k < n) ? count[i0 + k] : 0;
+    v[k] = 0;
+    if (i0 + k < n) {
+      v[k] = count[i0 + k];
+      count[i0 + k] = 0;
+    }
     tot += v[k];
   }
   sh[threadIdx.x] = tot;
@@ -274,65 +377,54 @@ __global__ void k_scan_blocks(const int32_t* count, int32_t* start, int32_t* bsu
   if (threadIdx.x == kScanBlock - 1) bsum[blockIdx.x] = sh[threadIdx.x];
 }
 
-__global__ void k_scan_sums(int32_t* bsum, int64_t nb, int32_t* start, int64_t n) {
-  __shared__ int32_t sh[1024];
-  int32_t carry = 0;
-  for (int64_t c0 = 0; c0 < nb; c0 += 1024) {
-    const int64_t i = c0 + threadIdx.x;
-    const int32_t v = (i < nb) ? bsum[i] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const int32_t add = (threadIdx.x >= (unsigned)off) ? sh[threadIdx.x - off] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += add;
-      __syncthreads();
-    }
-    if (i < nb) bsum[i] = carry + sh[threadIdx.x] - v;
-    const int32_t chunk_tot = sh[1023];
-    __syncthreads();
-    carry += chunk_tot;
-  }
-  if (threadIdx.x == 0) start[n] = carry;
+// Adds the bins of the scan blocks before this one (bsum) to the block's
+// local prefixes; the last block also writes the total start[n].
+__global__ __launch_bounds__(kScanBlock) void k_scan_add(int32_t* start, const int32_t* bsum,
+                                                         const int32_t* n_active, int TB) {
+  __shared__ int32_t red[kScanBlock];
+  const int64_t n = (int64_t)__builtin_amdgcn_readfirstlane(*n_active) * TB;
+  const int64_t blk = blockIdx.x;
+  if (blk * kScanBlock * kScanPer >= n) return;
+  const int32_t base = block_prefix(bsum, blk, red);
+  const int64_t i0 = (blk * kScanBlock + threadIdx.x) * kScanPer;
+  for (int k = 0; k < kScanPer; ++k)
+    if (i0 + k < n) start[i0 + k] += base;
+  if ((blk + 1) * kScanBlock * kScanPer >= n && threadIdx.x == 0) start[n] = base + bsum[blk];
 }
 
-__global__ void k_scan_add(int32_t* start, const int32_t* bsum, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) start[i] += bsum[i / (kScanBlock * kScanPer)];
-}
-
+// Bins in arbitrary order within each bin: placed[start[k] + slot[p]] = p.
 __global__ void k_place(const int32_t* key, const int32_t* slot, const int32_t* start,
-                        int32_t* sorted, int64_t n_slots) {
+                        int32_t* placed, int64_t n_slots) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_slots) return;
   const int32_t k = key[p];
   if (k < 0) {
-    sorted[p] = (int32_t)p;  // padding slot: the tail of its instance's range, in order
+    placed[p] = (int32_t)p;  // padding slot: the tail of its instance's range, in order
     return;
   }
-  sorted[start[k] + slot[p]] = (int32_t)p;
+  placed[start[k] + slot[p]] = (int32_t)p;
 }
 
-// Each bin sorted by slot (ascending), then rank[sorted[j]] = j.  Bins whose
-// entries already ascend (one run of adjacent lanes) are only checked, with
-// independent loads; the rest get an insertion sort.
-__global__ void k_bin_sort(const int32_t* start, int32_t* sorted, int32_t* rank, int64_t n_bins) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n_bins) return;
-  const int32_t s0 = start[b], s1 = start[b + 1];
-  bool ascending = true;
-  for (int32_t j = s0 + 1; j < s1; ++j) ascending &= sorted[j - 1] < sorted[j];
-  if (!ascending)
-  for (int32_t j = s0 + 1; j < s1; ++j) {
-    const int32_t v = sorted[j];
-    int32_t i = j - 1;
-    while (i >= s0 && sorted[i] > v) {
-      sorted[i + 1] = sorted[i];
-      --i;
-    }
-    sorted[i + 1] = v;
+// Stable order within each bin: the particle at placed position j goes to
+// start[k] + (number of its bin's particles with a smaller slot), so every
+// bin ends up ascending by slot; rank is the inverse permutation.  Work per
+// bin is quadratic in its size, with independent loads.
+__global__ __launch_bounds__(256) void k_bin_rank(const int32_t* key, const int32_t* start,
+                                                  const int32_t* placed, int32_t* sorted,
+                                                  int32_t* rank, int64_t n_slots) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_slots) return;
+  const int32_t v = placed[j];
+  const int32_t k = key[v];
+  int32_t r = (int32_t)j;
+  if (k >= 0) {
+    const int32_t s0 = start[k], s1 = start[k + 1];
+    r = s0;
+#pragma unroll 4
+    for (int32_t i = s0; i < s1; ++i) r += placed[i] < v;
   }
-  for (int32_t j = s0; j < s1; ++j) rank[sorted[j]] = j;
+  sorted[r] = v;
+  rank[v] = r;
 }
 
 // Record slot 15 carries the particle's cell as bits: (base+2) per axis in 10
@@ -396,22 +488,26 @@ __global__ __launch_bounds__(256) void k_prep_fwd(const mpm_instance* inst, cons
 // ---------------------------------------------------------------------------
 // P2G as a cell-binned segmented reduction into an LDS node tile.
 // Tile = TX*TY*TZ bins (contiguous keys), one thread each; node tile
-// (TX+2)(TY+2)(TZ+2).  partial: [n_tiles][4][TN] -- row 0 sum of weights
-// (MASS), rows 1-3 sum of w (Q + A' o).  Empty tiles return at once and are
-// skipped by the node kernels.
+// (TX+2)(TY+2)(TZ+2).  One block per occupied unit of this step (blocks past
+// n_active and dead units return at once).  partial: [active unit][4][TN] --
+// row 0 sum of weights (MASS), rows 1-3 sum of w (Q + A' o).
 // ---------------------------------------------------------------------------
 template <typename T, int TX, int TY, int TZ, bool MASS>
 __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* inst,
-                                                        const int32_t* tile_inst,
+                                                        const int32_t* unit_inst,
+                                                        const int32_t* active_unit,
+                                                        const int32_t* n_active,
                                                         const int32_t* bin_start,
                                                         const T* payload, T* partial) {
   constexpr int NY = TY + 2, NZ = TZ + 2, TN = (TX + 2) * NY * NZ;
   constexpr int NT = TX * TY * TZ;
   __shared__ T lds[4 * TN];
-  const int64_t tile = blockIdx.x;
-  const mpm_instance& I = inst[__builtin_amdgcn_readfirstlane(tile_inst[tile])];
-  const int64_t key0 = I.c_off + (tile - I.t_off) * NT;
-  if (bin_start[key0] == bin_start[key0 + NT]) return;  // empty tile
+  const int64_t a = blockIdx.x;
+  if (a >= *n_active) return;
+  const int32_t u = active_unit[a];
+  const mpm_instance& I = inst[__builtin_amdgcn_readfirstlane(unit_inst[u])];
+  if (u == dead_unit(I)) return;
+  const int64_t key0 = a * NT;
   const int tid = threadIdx.x;
   const int lz = tid % TZ, ly = (tid / TZ) % TY, lx = tid / (TY * TZ);
 
@@ -464,16 +560,16 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
     lds[3 * TN + nl] += am[o][2];
   }
   __syncthreads();
-  T* out = partial + tile * (int64_t)(4 * TN);
+  T* out = partial + a * (int64_t)(4 * TN);
   for (int i = tid; i < 4 * TN; i += NT) out[i] = lds[i];
 }
 
-// Sum of the non-empty tile partials covering node (i, j, k), tiles in
+// Sum of the occupied tile partials covering node (i, j, k), tiles in
 // ascending order.
 template <typename T, int TX, int TY, int TZ>
-__device__ __forceinline__ void gather_tiles(const Inst<T>& I, const int32_t* bin_start,
+__device__ __forceinline__ void gather_tiles(const Inst<T>& I, const int32_t* unit_rank,
                                              const T* partial, int i, int j, int k, T acc[4]) {
-  constexpr int NY = TY + 2, NZ = TZ + 2, TN = (TX + 2) * NY * NZ, NT = TX * TY * TZ;
+  constexpr int NY = TY + 2, NZ = TZ + 2, TN = (TX + 2) * NY * NZ;
   acc[0] = acc[1] = acc[2] = acc[3] = T(0);
   // tile ix covers nodes [ix TX - sx - 2, ix TX - sx + TX - 1]
   const int xhi = min((i + 2 + I.sx) / TX, I.tx - 1), yhi = min((j + 2 + I.sy) / TY, I.ty - 1),
@@ -488,9 +584,9 @@ __device__ __forceinline__ void gather_tiles(const Inst<T>& I, const int32_t* bi
         const int lk = k - (iz * TZ - I.sz - 2);
         if (lk < 0 || lk >= TZ + 2) continue;
         const int64_t lt = ((int64_t)ix * I.ty + iy) * I.tz + iz;
-        const int64_t key0 = I.c_off + lt * NT;
-        if (bin_start[key0] == bin_start[key0 + NT]) continue;
-        const T* src = partial + (I.t_off + lt) * (int64_t)(4 * TN) + (li * NY + lj) * NZ + lk;
+        const int32_t a = unit_rank[I.t_off + lt];
+        if (a < 0) continue;
+        const T* src = partial + a * (int64_t)(4 * TN) + (li * NY + lj) * NZ + lk;
         acc[0] += src[0];
         acc[1] += src[TN];
         acc[2] += src[2 * TN];
@@ -504,7 +600,7 @@ __device__ __forceinline__ void gather_tiles(const Inst<T>& I, const int32_t* bi
 // grid record per node: (m, vx, vy, vz).
 template <typename T, int TX, int TY, int TZ>
 __global__ __launch_bounds__(256) void k_grid_fwd(const mpm_instance* inst, const int32_t* node_inst,
-                                                  int64_t n_nodes, const int32_t* bin_start,
+                                                  int64_t n_nodes, const int32_t* unit_rank,
                                                   const T* partial, T* grid) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_nodes) return;
@@ -514,7 +610,7 @@ __global__ __launch_bounds__(256) void k_grid_fwd(const mpm_instance* inst, cons
   const int k = (int)(local % I.nz), j = (int)((local / I.nz) % I.ny),
             i = (int)(local / ((int64_t)I.nz * I.ny));
   T acc[4];
-  gather_tiles<T, TX, TY, TZ>(I, bin_start, partial, i, j, k, acc);
+  gather_tiles<T, TX, TY, TZ>(I, unit_rank, partial, i, j, k, acc);
   const T m = I.pmass * acc[0];
   T v[3] = {T(0), T(0), T(0)};
   if (m > T(kMassEps))
@@ -614,7 +710,7 @@ __global__ __launch_bounds__(256) void k_prep_adj(const mpm_instance* inst, cons
 //   gadj record per node: (gm_bar, gmom_bar x, y, z).
 template <typename T, int TX, int TY, int TZ>
 __global__ __launch_bounds__(256) void k_grid_adj(const mpm_instance* inst, const int32_t* node_inst,
-                                                  int64_t n_nodes, const int32_t* bin_start,
+                                                  int64_t n_nodes, const int32_t* unit_rank,
                                                   const T* partial, const T* grid, T* gadj,
                                                   int advect) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -625,7 +721,7 @@ __global__ __launch_bounds__(256) void k_grid_adj(const mpm_instance* inst, cons
   const int k = (int)(local % I.nz), j = (int)((local / I.nz) % I.ny),
             i = (int)(local / ((int64_t)I.nz * I.ny));
   T acc[4];
-  gather_tiles<T, TX, TY, TZ>(I, bin_start, partial, i, j, k, acc);
+  gather_tiles<T, TX, TY, TZ>(I, unit_rank, partial, i, j, k, acc);
   const V4<T> gr = ld4(grid + 4 * g);
   const T m = gr.x;
   T mb[3] = {T(0), T(0), T(0)};
@@ -876,8 +972,24 @@ int64_t tile_nodes(const mpm_batch* b) {
 inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
 struct Workspace {
-  int32_t *key, *slot, *count, *bsum;
+  int32_t *key, *slot, *placed, *count, *bsum, *unit_flag, *ucount;
   void *payload, *partial, *gadj, *md;
+};
+
+// Views into a step's sort record:
+// rank[n_slots] | sorted[n_slots] | start[n_bins + 1] | unit_rank[n_tiles] |
+// active_unit[n_tiles] | n_active[1].
+template <typename I32>
+struct SortView {
+  I32 *rank, *sorted, *start, *unit_rank, *active_unit, *n_active;
+  SortView(const mpm_batch* b, I32* s) {
+    rank = s;
+    sorted = s + b->n_slots;
+    start = s + 2 * b->n_slots;
+    unit_rank = start + b->n_bins + 1;
+    active_unit = unit_rank + b->n_tiles;
+    n_active = active_unit + b->n_tiles;
+  }
 };
 
 int64_t scan_blocks(int64_t n_bins) {
@@ -896,6 +1008,9 @@ int64_t carve(const mpm_batch* b, size_t tsz, char* base, Workspace* w) {
   ws.slot = (int32_t*)take(4 * b->n_slots);
   ws.count = (int32_t*)take(4 * b->n_bins);
   ws.bsum = (int32_t*)take(4 * (scan_blocks(b->n_bins) + 1));
+  ws.unit_flag = (int32_t*)take(4 * b->n_tiles);
+  ws.ucount = (int32_t*)take(4 * ((b->n_tiles + 255) / 256));
+  ws.placed = (int32_t*)take(4 * b->n_slots);
   ws.payload = take((int64_t)tsz * kRecord * b->n_slots);
   ws.partial = take((int64_t)tsz * 4 * tile_nodes(b) * b->n_tiles);
   ws.gadj = take((int64_t)tsz * 4 * b->n_nodes);
@@ -912,7 +1027,11 @@ int validate(const mpm_batch* b) {
     return set_err(MPM_EINVAL, "n_slots / n_nodes must be positive multiples of %d", kChunk);
   if (b->n_slots >= (int64_t)1 << 31 || b->n_bins >= (int64_t)1 << 31)
     return set_err(MPM_EINVAL, "n_slots and n_bins must be below 2^31");
-  if (b->n_bins <= 0 || b->n_tiles <= 0) return set_err(MPM_EINVAL, "empty bin or tile table");
+  if (b->n_tiles <= 0 || b->n_tiles < b->n_inst)
+    return set_err(MPM_EINVAL, "unit table must hold every instance's tiles and dead unit");
+  if (tile_index(b) >= 0 &&
+      b->n_bins != b->n_tiles * (int64_t)b->tile[0] * b->tile[1] * b->tile[2])
+    return set_err(MPM_EINVAL, "n_bins must be n_tiles * tile cells");
   if (b->block != 64 && b->block != 128 && b->block != 256)
     return set_err(MPM_EINVAL, "block must be 64, 128 or 256 (got %d)", b->block);
   if (b->n_inst > 0 && b->inst == nullptr) return set_err(MPM_EINVAL, "batch tables missing");
@@ -972,69 +1091,78 @@ struct TimedLaunch {
 template <typename T>
 int sort_impl(const mpm_batch* b, const T* dyn, int32_t* sortbuf, const Workspace& w,
               hipStream_t st) {
-  int32_t* rank = sortbuf;
-  int32_t* sorted = sortbuf + b->n_slots;
-  int32_t* start = sortbuf + 2 * b->n_slots;
+  const SortView<int32_t> v(b, sortbuf);
   const int bs = b->block;
-  TimedLaunch tl(MPM_K_SORT, st);
-  k_zero_i32<<<grid1(b->n_bins, 256), 256, 0, st>>>(w.count, b->n_bins);
-  k_sort_key<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn,
-                                                       b->tile[0], b->tile[1], b->tile[2], w.key,
-                                                       w.slot, w.count);
+  const int TB = b->tile[0] * b->tile[1] * b->tile[2];
   const int64_t nb = scan_blocks(b->n_bins);
-  k_scan_blocks<<<dim3((unsigned)nb), kScanBlock, 0, st>>>(w.count, start, w.bsum, b->n_bins);
-  k_scan_sums<<<1, 1024, 0, st>>>(w.bsum, nb, start, b->n_bins);
-  k_scan_add<<<grid1(b->n_bins, 256), 256, 0, st>>>(start, w.bsum, b->n_bins);
-  k_place<<<grid1(b->n_slots, bs), bs, 0, st>>>(w.key, w.slot, start, sorted, b->n_slots);
-  k_bin_sort<<<grid1(b->n_bins, 64), 64, 0, st>>>(start, sorted, rank, b->n_bins);
+  const int64_t nu = (b->n_tiles + 255) / 256;
+  TimedLaunch tl(MPM_K_SORT, st);
+  k_unit_mark<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn,
+                                                        b->tile[0], b->tile[1], b->tile[2],
+                                                        w.unit_flag);
+  k_unit_count<<<dim3((unsigned)nu), 256, 0, st>>>(b->tile_inst, b->n_tiles, w.unit_flag,
+                                                   w.ucount);
+  k_unit_rank<<<dim3((unsigned)nu), 256, 0, st>>>(b->tile_inst, b->n_tiles, w.unit_flag, w.ucount,
+                                                  v.unit_rank, v.active_unit, v.n_active);
+  k_sort_key<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn,
+                                                       b->tile[0], b->tile[1], b->tile[2],
+                                                       v.unit_rank, w.key, w.slot, w.count);
+  k_scan_blocks<<<dim3((unsigned)nb), kScanBlock, 0, st>>>(w.count, v.start, w.bsum, v.n_active,
+                                                            TB);
+  k_scan_add<<<dim3((unsigned)nb), kScanBlock, 0, st>>>(v.start, w.bsum, v.n_active, TB);
+  k_place<<<grid1(b->n_slots, bs), bs, 0, st>>>(w.key, w.slot, v.start, w.placed, b->n_slots);
+  k_bin_rank<<<grid1(b->n_slots, 256), 256, 0, st>>>(w.key, v.start, w.placed, v.sorted, v.rank,
+                                                     b->n_slots);
   return check_launch("sort_by_cell");
 }
 
 template <typename T, int TX, int TY, int TZ>
-void launch_tile_fwd(const mpm_batch* b, const int32_t* start, const Workspace& w,
+void launch_tile_fwd(const mpm_batch* b, const SortView<const int32_t>& v, const Workspace& w,
                      const T* payload, T* grid, hipStream_t st) {
   {
   TimedLaunch tl(MPM_K_P2G_FWD, st);
   k_p2g_tile<T, TX, TY, TZ, true><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
-      b->inst, b->tile_inst, start, payload, (T*)w.partial);
+      b->inst, b->tile_inst, v.active_unit, v.n_active, v.start, payload, (T*)w.partial);
   }
   TimedLaunch tl(MPM_K_GRID_FWD, st);
   k_grid_fwd<T, TX, TY, TZ><<<grid1(b->n_nodes, 256), 256, 0, st>>>(
-      b->inst, b->node_inst, b->n_nodes, start, (const T*)w.partial, grid);
+      b->inst, b->node_inst, b->n_nodes, v.unit_rank, (const T*)w.partial, grid);
 }
 
 template <typename T, int TX, int TY, int TZ>
-void launch_tile_adj(const mpm_batch* b, const int32_t* start, const Workspace& w, const T* grid,
-                     int advect, hipStream_t st) {
+void launch_tile_adj(const mpm_batch* b, const SortView<const int32_t>& v, const Workspace& w,
+                     const T* grid, int advect, hipStream_t st) {
   {
   TimedLaunch tl(MPM_K_P2G_ADJ_TILE, st);
   k_p2g_tile<T, TX, TY, TZ, false><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
-      b->inst, b->tile_inst, start, (const T*)w.payload, (T*)w.partial);
+      b->inst, b->tile_inst, v.active_unit, v.n_active, v.start, (const T*)w.payload,
+      (T*)w.partial);
   }
   TimedLaunch tl(MPM_K_GRID_ADJ, st);
   k_grid_adj<T, TX, TY, TZ><<<grid1(b->n_nodes, 256), 256, 0, st>>>(
-      b->inst, b->node_inst, b->n_nodes, start, (const T*)w.partial, grid, (T*)w.gadj, advect);
+      b->inst, b->node_inst, b->n_nodes, v.unit_rank, (const T*)w.partial, grid, (T*)w.gadj,
+      advect);
 }
 
 template <typename T>
-void dispatch_fwd(int ti, const mpm_batch* b, const int32_t* start, const Workspace& w,
-                  const T* payload, T* grid, hipStream_t st) {
+void dispatch_fwd(int ti, const mpm_batch* b, const SortView<const int32_t>& sv,
+                  const Workspace& w, const T* payload, T* grid, hipStream_t st) {
   switch (ti) {
-    case 0: launch_tile_fwd<T, 4, 8, 8>(b, start, w, payload, grid, st); break;
-    case 1: launch_tile_fwd<T, 4, 4, 16>(b, start, w, payload, grid, st); break;
-    case 2: launch_tile_fwd<T, 8, 8, 4>(b, start, w, payload, grid, st); break;
-    default: launch_tile_fwd<T, 4, 4, 8>(b, start, w, payload, grid, st); break;
+    case 0: launch_tile_fwd<T, 4, 8, 8>(b, sv, w, payload, grid, st); break;
+    case 1: launch_tile_fwd<T, 4, 4, 16>(b, sv, w, payload, grid, st); break;
+    case 2: launch_tile_fwd<T, 8, 8, 4>(b, sv, w, payload, grid, st); break;
+    default: launch_tile_fwd<T, 4, 4, 8>(b, sv, w, payload, grid, st); break;
   }
 }
 
 template <typename T>
-void dispatch_adj(int ti, const mpm_batch* b, const int32_t* start, const Workspace& w,
-                  const T* grid, int advect, hipStream_t st) {
+void dispatch_adj(int ti, const mpm_batch* b, const SortView<const int32_t>& sv,
+                  const Workspace& w, const T* grid, int advect, hipStream_t st) {
   switch (ti) {
-    case 0: launch_tile_adj<T, 4, 8, 8>(b, start, w, grid, advect, st); break;
-    case 1: launch_tile_adj<T, 4, 4, 16>(b, start, w, grid, advect, st); break;
-    case 2: launch_tile_adj<T, 8, 8, 4>(b, start, w, grid, advect, st); break;
-    default: launch_tile_adj<T, 4, 4, 8>(b, start, w, grid, advect, st); break;
+    case 0: launch_tile_adj<T, 4, 8, 8>(b, sv, w, grid, advect, st); break;
+    case 1: launch_tile_adj<T, 4, 4, 16>(b, sv, w, grid, advect, st); break;
+    case 2: launch_tile_adj<T, 8, 8, 4>(b, sv, w, grid, advect, st); break;
+    default: launch_tile_adj<T, 4, 4, 8>(b, sv, w, grid, advect, st); break;
   }
 }
 
@@ -1051,8 +1179,8 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   carve(b, sizeof(T), (char*)ws, &w);
   rc = sort_impl<T>(b, dyn_in, sortbuf, w, st);
   if (rc) return rc;
-  const int32_t* rank = sortbuf;
-  const int32_t* start = sortbuf + 2 * b->n_slots;
+  const SortView<const int32_t> sv(b, sortbuf);
+  const int32_t* rank = sv.rank;
   const int bs = b->block;
   {
   TimedLaunch tl(MPM_K_PREP_FWD, st);
@@ -1061,7 +1189,7 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
         b->inst, b->slot_inst, b->n_slots, dyn_in, Fg, sched, rank, payload);
   });
   }
-  dispatch_fwd<T>(tile_index(b), b, start, w, payload, grid, st);
+  dispatch_fwd<T>(tile_index(b), b, sv, w, payload, grid, st);
   TimedLaunch tl(MPM_K_G2P, st);
   k_g2p<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in, grid,
                                                  dyn_out, advect);
@@ -1082,16 +1210,16 @@ int step_bwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   hipStream_t st = (hipStream_t)stream;
   Workspace w;
   carve(b, sizeof(T), (char*)ws, &w);
-  const int32_t* rank = sortbuf;
-  const int32_t* start = sortbuf + 2 * b->n_slots;
+  const SortView<const int32_t> sv(b, sortbuf);
+  const int32_t* rank = sv.rank;
   const int bs = b->block;
   {
   TimedLaunch tl(MPM_K_PREP_ADJ, st);
   k_prep_adj<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in,
                                                       bar_out, rank, (T*)w.payload, advect);
   }
-  dispatch_adj<T>(tile_index(b), b, start, w, grid, advect, st);
-  const int32_t* sorted = sortbuf + b->n_slots;
+  dispatch_adj<T>(tile_index(b), b, sv, w, grid, advect, st);
+  const int32_t* sorted = sv.sorted;
   {
   TimedLaunch tl(MPM_K_P2G_ADJ, st);
   k_adj_gather<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(

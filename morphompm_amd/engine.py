@@ -90,7 +90,7 @@ class BatchPlan:
             raise ValueError(f"tile {self.tile} not built; available {_lib.tile_shapes()}")
         insts = (MpmInstance * len(self.specs))()
         slot_inst, node_inst, tile_inst = [], [], []
-        p_off = g_off = c_off = t_off = 0
+        p_off = g_off = t_off = 0
         self.p_off, self.g_off = [], []
         for i, s in enumerate(self.specs):
             nx, ny, nz = s.shape
@@ -100,7 +100,7 @@ class BatchPlan:
             I = insts[i]
             I.shift[0], I.shift[1], I.shift[2] = sh
             I.origin[0], I.origin[1], I.origin[2] = (int(o) for o in s.origin)
-            I.p_off, I.g_off, I.c_off, I.t_off = p_off, g_off, c_off, t_off
+            I.p_off, I.g_off, I.t_off = p_off, g_off, t_off
             I.n, I.nx, I.ny, I.nz, I.tx, I.ty, I.tz = s.n, nx, ny, nz, tx, ty, tz
             I.model = s.kind
             I.dx, I.dt, I.damping, I.pmass, I.vol0 = s.dx, s.dt, s.damping, s.pmass, s.vol0
@@ -112,12 +112,12 @@ class BatchPlan:
             gs = _ceil(nx * ny * nz, SLOT_ALIGN) * SLOT_ALIGN
             slot_inst += [i] * (ps // SLOT_ALIGN)
             node_inst += [i] * (gs // SLOT_ALIGN)
-            tile_inst += [i] * (tx * ty * tz)
+            tile_inst += [i] * (tx * ty * tz + 1)   # tiles, then the dead unit
             p_off += ps
             g_off += gs
-            c_off += tx * ty * tz * int(np.prod(self.tile)) + 1
-            t_off += tx * ty * tz
-        self.n_slots, self.n_nodes, self.n_bins, self.n_tiles = p_off, g_off, c_off, t_off
+            t_off += tx * ty * tz + 1
+        self.n_slots, self.n_nodes, self.n_tiles = p_off, g_off, t_off
+        self.n_bins = t_off * int(np.prod(self.tile))
         self.n_inst = len(self.specs)
         self._insts = insts
         raw = bytes(insts)
@@ -137,7 +137,7 @@ class BatchPlan:
         c.node_inst = self.d_node_inst.data_ptr()
         c.tile_inst = self.d_tile_inst.data_ptr()
         self.c = c
-        self.sortlen = 2 * self.n_slots + self.n_bins + 1
+        self.sortlen = 2 * self.n_slots + self.n_bins + 1 + 2 * self.n_tiles + 1
         self._ws = {}
         # slot -> live mask and per-chunk instance (host copies for packing)
         self.slot_inst_np = np.repeat(np.array(slot_inst, dtype=np.int64), SLOT_ALIGN)
