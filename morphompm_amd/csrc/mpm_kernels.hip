@@ -22,6 +22,7 @@
 // Every sum has a fixed order, so results are bitwise reproducible.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -621,6 +622,131 @@ __global__ __launch_bounds__(256) void k_grid_fwd(const mpm_instance* inst, cons
 // ---------------------------------------------------------------------------
 // G2P (transfer.py:61-77)
 // ---------------------------------------------------------------------------
+// Stencil nodes of a sorted 256-slot chunk, staged in LDS.  Particles in
+// sorted order are spatially compact, so the box spanned by the chunk's
+// stencils (base .. base+2 per axis over its particles with an in-grid node)
+// is small (a few hundred nodes).  When it exceeds the LDS capacity the
+// chunk reads its nodes from the grid in global memory instead.  Nodes
+// outside the grid read as zero either way.
+template <typename T>
+struct BoxCap {
+  static constexpr int value = sizeof(T) == 4 ? 1024 : 640;
+};
+
+struct NodeBox {
+  int lo[3], d[3];
+  int vol;
+  bool staged;
+};
+
+// Block-wide bounding box of the valid particles' bases (blockDim 256).
+template <int CAP>
+__device__ __forceinline__ NodeBox chunk_box(bool valid, const int base[3], int* red) {
+  int v[6];
+  for (int c = 0; c < 3; ++c) {
+    v[c] = valid ? base[c] : INT_MAX;
+    v[3 + c] = valid ? base[c] : INT_MIN;
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int c = 0; c < 3; ++c) {
+      v[c] = min(v[c], __shfl_xor(v[c], off));
+      v[3 + c] = max(v[3 + c], __shfl_xor(v[3 + c], off));
+    }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int c = 0; c < 6; ++c) red[wave * 6 + c] = v[c];
+  __syncthreads();
+  NodeBox b;
+  int hi[3];
+  for (int c = 0; c < 3; ++c) {
+    b.lo[c] = min(min(red[c], red[6 + c]), min(red[12 + c], red[18 + c]));
+    hi[c] = max(max(red[3 + c], red[9 + c]), max(red[15 + c], red[21 + c]));
+  }
+  const bool any = hi[0] >= b.lo[0];
+  for (int c = 0; c < 3; ++c) b.d[c] = any ? hi[c] - b.lo[c] + 3 : 0;
+  b.vol = b.d[0] * b.d[1] * b.d[2];
+  b.staged = any && b.vol <= CAP;
+  return b;
+}
+
+// lds[n] = record of box node n (x-major), zero outside the grid.
+template <typename T>
+__device__ __forceinline__ void stage_box(const Inst<T>& I, const NodeBox& b, const T* rec,
+                                          V4<T>* lds) {
+  for (int n = threadIdx.x; n < b.vol; n += 256) {
+    const int k = n % b.d[2], j = (n / b.d[2]) % b.d[1], i = n / (b.d[1] * b.d[2]);
+    const int gx = b.lo[0] + i, gy = b.lo[1] + j, gz = b.lo[2] + k;
+    V4<T> g = {T(0), T(0), T(0), T(0)};
+    if (gx >= 0 && gx < I.nx && gy >= 0 && gy < I.ny && gz >= 0 && gz < I.nz)
+      g = ld4(rec + 4 * (I.g_off + ((int64_t)gx * I.ny + gy) * I.nz + gz));
+    lds[n] = g;
+  }
+}
+
+__device__ __forceinline__ bool base_in_grid(const int base[3], int nx, int ny, int nz) {
+  return base[0] >= -2 && base[0] < nx && base[1] >= -2 && base[1] < ny && base[2] >= -2 &&
+         base[2] < nz;
+}
+
+// Node records of one particle's 3x3x3 stencil: from the staged box, or from
+// global memory (zero outside the grid).
+template <typename T>
+struct BoxNodes {
+  const V4<T>* lds;
+  int l0, sy, sz;
+  __device__ __forceinline__ V4<T> operator()(int i, int j, int k) const {
+    return lds[l0 + i * sy + j * sz + k];
+  }
+};
+template <typename T>
+struct GridNodes {
+  const T* rec;
+  int64_t g_off;
+  int ny, nz;
+  AxisNodes ax, ay, az;
+  // select, not array indexing: i, j may be loop variables of a rolled loop
+  template <typename E>
+  static __device__ __forceinline__ E pick(const E a[3], int i) {
+    return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
+  }
+  __device__ __forceinline__ V4<T> operator()(int i, int j, int k) const {
+    const int64_t gi =
+        g_off + ((int64_t)pick(ax.idx, i) * ny + pick(ay.idx, j)) * nz + pick(az.idx, k);
+    const V4<T> g = ld4(rec + 4 * gi);
+    const bool ok = pick(ax.ok, i) && pick(ay.ok, j) && pick(az.ok, k);
+    const V4<T> z = {T(0), T(0), T(0), T(0)};
+    return ok ? g : z;
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ BoxNodes<T> box_nodes(const V4<T>* lds, const NodeBox& b,
+                                                 const int base[3]) {
+  BoxNodes<T> s;
+  s.lds = lds;
+  s.sz = b.d[2];
+  s.sy = b.d[1] * b.d[2];
+  s.l0 = (base[0] - b.lo[0]) * s.sy + (base[1] - b.lo[1]) * s.sz + (base[2] - b.lo[2]);
+  return s;
+}
+template <typename T>
+__device__ __forceinline__ GridNodes<T> grid_nodes(const T* rec, const Inst<T>& I,
+                                                   const int base[3]) {
+  GridNodes<T> s;
+  s.rec = rec;
+  s.g_off = I.g_off;
+  s.ny = I.ny;
+  s.nz = I.nz;
+  s.ax = axis_nodes(base[0], I.nx);
+  s.ay = axis_nodes(base[1], I.ny);
+  s.az = axis_nodes(base[2], I.nz);
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// G2P (transfer.py:61-77), one thread per particle slot (coalesced state
+// reads and writes); stencil nodes come from the grid through the caches.
+// ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int32_t* slot_inst,
                                              int64_t n_slots, const T* dyn_in, const T* grid,
@@ -633,36 +759,54 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
   int base[3];
   ld3(dyn_in, n_slots, p, RX, x);
   stencil_base(x, I, base, fx);
-  const AxisNodes nx = axis_nodes(base[0], I.nx), ny = axis_nodes(base[1], I.ny),
-                  nz = axis_nodes(base[2], I.nz);
-  T w[3][3];
-  for (int a = 0; a < 3; ++a) bspline_w(fx[a], w[a]);
+  const bool valid = base_in_grid(base, I.nx, I.ny, I.nz);
+  // Separable weights: with d = (o - fx) dx,
+  //   nv = sum w v,   nC = k4 dx (sum w v o^T - nv fx^T),
+  // and each z-row (i, j) of 3 nodes is summed first.
   T nv[3] = {T(0), T(0), T(0)};
   M3<T> nC = zero3<T>();
+  auto gather = [&](const auto& nodes) {
+    T w[3][3];
+    for (int c = 0; c < 3; ++c) bspline_w(fx[c], w[c]);
+    const T wz0 = w[2][0], wz1 = w[2][1], wz2 = w[2][2], wz2k = T(2) * w[2][2];
+    T Si[3] = {T(0), T(0), T(0)}, Sj[3] = {T(0), T(0), T(0)}, Sk[3] = {T(0), T(0), T(0)};
+    // x planes stay rolled to bound the gathered values in flight
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+      const T wxi = i == 0 ? w[0][0] : (i == 1 ? w[0][1] : w[0][2]);
+      const T Ti = T(i);
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const int64_t gi = I.g_off + ((int64_t)nx.idx[i] * I.ny + ny.idx[j]) * I.nz + nz.idx[k];
-        const V4<T> g = ld4(grid + 4 * gi);
-        const bool ok = nx.ok[i] && ny.ok[j] && nz.ok[k];
-        const T vi[3] = {ok ? g.y : T(0), ok ? g.z : T(0), ok ? g.w : T(0)};
-        const T wt = w[0][i] * w[1][j] * w[2][k];
-        const T d[3] = {(T(i) - fx[0]) * I.dx, (T(j) - fx[1]) * I.dx, (T(k) - fx[2]) * I.dx};
-        const T sw = I.k4 * wt;
-        for (int a = 0; a < 3; ++a) {
-          nv[a] += wt * vi[a];
-          for (int b = 0; b < 3; ++b) nC(a, b) += sw * (vi[a] * d[b]);
+      for (int jj = 0; jj < 3; ++jj) {
+        const V4<T> g0 = nodes(i, jj, 0), g1 = nodes(i, jj, 1), g2 = nodes(i, jj, 2);
+        const T v0[3] = {g0.y, g0.z, g0.w}, v1[3] = {g1.y, g1.z, g1.w},
+                v2[3] = {g2.y, g2.z, g2.w};
+        const T wxy = wxi * w[1][jj];
+        for (int c = 0; c < 3; ++c) {
+          const T r0 = wz0 * v0[c] + wz1 * v1[c] + wz2 * v2[c];
+          const T r1 = wz1 * v1[c] + wz2k * v2[c];
+          nv[c] += wxy * r0;
+          Sk[c] += wxy * r1;
+          Si[c] += (Ti * wxy) * r0;
+          if (jj) Sj[c] += (T(jj) * wxy) * r0;
         }
       }
+    }
+    const T s = I.k4 * I.dx;
+    for (int c = 0; c < 3; ++c) {
+      nC(c, 0) = s * (Si[c] - nv[c] * fx[0]);
+      nC(c, 1) = s * (Sj[c] - nv[c] * fx[1]);
+      nC(c, 2) = s * (Sk[c] - nv[c] * fx[2]);
+    }
+  };
+  if (valid) gather(grid_nodes(grid, I, base));
   const M3<T> F = ld9(dyn_in, n_slots, p, RF);
-  M3<T> A = scaled(nC, I.dt);
-  A.a[0] += T(1); A.a[4] += T(1); A.a[8] += T(1);
-  const M3<T> Fn = mul(A, F);
+  // F_new = (I + dt C) F, formed as F + dt (C F): I + dt C would round away
+  // the low bits of the small increment the same way for every particle.
+  const M3<T> CF = mul(nC, F);
+  M3<T> Fn;
+  for (int k = 0; k < 9; ++k) Fn.a[k] = F.a[k] + I.dt * CF.a[k];
   T xn[3];
-  for (int a = 0; a < 3; ++a) xn[a] = advect ? x[a] + I.dt * nv[a] : x[a];
+  for (int c = 0; c < 3; ++c) xn[c] = advect ? x[c] + I.dt * nv[c] : x[c];
   st3(dyn_out, n_slots, p, RX, xn);
   st3(dyn_out, n_slots, p, RV, nv);
   st9(dyn_out, n_slots, p, RC, nC);
@@ -734,10 +878,11 @@ __global__ __launch_bounds__(256) void k_grid_adj(const mpm_instance* inst, cons
 }
 
 // reverse step, part 3a (transfer.py:100-119 x_bar terms, :129-147): the
-// P2G-adjoint gather, one thread per sorted slot j (neighbouring lanes share
-// stencil nodes).  Reads the forward payload pf (fx, q2, A', cell) and the
-// adjoint payload pa (fx, q1, P1) of the particle in slot j.  With dpos =
-// (o - fx) dx for stencil offset o, the per-node factors are affine in o:
+// P2G-adjoint gather, one thread per sorted slot j; each 256-slot chunk
+// stages its stencil box of the grid adjoint (and, with advection, of the
+// grid) in LDS.  Reads the forward payload pf (fx, q2, A', cell) and the
+// adjoint payload pa (fx, q1, P1) of slot j.  With dpos = (o - fx) dx for
+// stencil offset o, the per-node factors are affine in o:
 //   nv_b + 4 inv_dx^2 C_b' dpos = q1 + P1 o   (P1 = 4 inv_dx^2 dx C_b')
 //   pmass v + A dpos            = q2 + A' o   (A' = A dx, A = pmass C + dt c tau)
 // so x_bar's grad-w coefficient is v_i.(q1 + P1 o) + mb_i.(q2 + A' o) + gm_b,i pmass.
@@ -749,17 +894,20 @@ __global__ __launch_bounds__(256) void k_adj_gather(const mpm_instance* inst,
                                                     const int32_t* sorted, const T* pf, const T* pa,
                                                     const T* grid, const T* gadj, const T* bar_out,
                                                     T* bar_in, T* md, int advect) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_slots) return;
+  constexpr int CAP = BoxCap<T>::value;
+  __shared__ V4<T> lg[CAP];  // grid adjoint (gm_b, mb)
+  __shared__ V4<T> lv[CAP];  // grid (m, v), advection only
+  __shared__ int red[24];
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;  // n_slots is a multiple of 256
   const Inst<T> I = load_inst<T>(inst, block_inst(slot_inst));
   const int64_t p = sorted[j];
-  if (p - I.p_off >= I.n) return;
-  const T* rf = pf + j * kRecord;
-  const T* ra = pa + j * kRecord;
-  T fx[3], q1[3], q2[3];
+  const bool live = p - I.p_off < I.n;
+  T fx[3] = {T(0), T(0), T(0)}, q1[3], q2[3];
   M3<T> P1, Ap;
-  uint32_t cell;
-  {
+  uint32_t cell = kNoCell;
+  if (live) {
+    const T* rf = pf + j * kRecord;
+    const T* ra = pa + j * kRecord;
     const V4<T> f0 = ld4(rf), f1 = ld4(rf + 4), f2 = ld4(rf + 8), f3 = ld4(rf + 12);
     fx[0] = f0.x; fx[1] = f0.y; fx[2] = f0.z;
     q2[0] = f0.w; q2[1] = f1.x; q2[2] = f1.y;
@@ -771,22 +919,34 @@ __global__ __launch_bounds__(256) void k_adj_gather(const mpm_instance* inst,
     P1.a[0] = a1.z; P1.a[1] = a1.w; P1.a[2] = a2.x; P1.a[3] = a2.y; P1.a[4] = a2.z;
     P1.a[5] = a2.w; P1.a[6] = a3.x; P1.a[7] = a3.y; P1.a[8] = a3.z;
   }
+  const bool valid = live && cell != kNoCell;
+  const int base[3] = {(int)(cell & 1023u) - 2, (int)((cell >> 10) & 1023u) - 2,
+                       (int)(cell >> 20) - 2};
+  const NodeBox bx = chunk_box<CAP>(valid, base, red);
+  if (bx.staged) {
+    stage_box(I, bx, gadj, lg);
+    if (advect) stage_box(I, bx, grid, lv);
+  }
+  __syncthreads();
+  if (!live) return;
   T xb[3] = {T(0), T(0), T(0)};
   T vb[3] = {T(0), T(0), T(0)};
   M3<T> MD = zero3<T>();
   T Sv[3] = {T(0), T(0), T(0)};  // sum w v_i
-  if (cell != kNoCell) {
-    const int base[3] = {(int)(cell & 1023u) - 2, (int)((cell >> 10) & 1023u) - 2,
-                         (int)(cell >> 20) - 2};
-    const AxisNodes nx = axis_nodes(base[0], I.nx), ny = axis_nodes(base[1], I.ny),
-                    nz = axis_nodes(base[2], I.nz);
+  // Separable weights: with d = (o - fx) dx, vb = sum w mb and
+  //   MD = dx (sum w mb o^T - vb fx^T);
+  // x_bar's terms sum alpha grad w, grad w = (dwx wy wz, wx dwy wz, wx wy dwz) / dx,
+  // over each z-row (i, j) of 3 nodes first.
+  M3<T> Mo = zero3<T>();  // sum w mb o^T
+  auto gather = [&](const auto& adj_nodes, const auto& vel_nodes) {
     T w[3][3], dw[3][3];
-    for (int a = 0; a < 3; ++a) {
-      bspline_w(fx[a], w[a]);
-      bspline_dw(fx[a], dw[a]);
+    for (int c = 0; c < 3; ++c) {
+      bspline_w(fx[c], w[c]);
+      bspline_dw(fx[c], dw[c]);
     }
-    // The (i, j) loops stay rolled (9 rows of 3 nodes) to bound the gathered
-    // values in flight; per-axis values of row (i, j) are picked by select.
+    const T wz0 = w[2][0], wz1 = w[2][1], wz2 = w[2][2], wz2k = T(2) * w[2][2];
+    // The (i, j) loop stays rolled (9 rows of 3 nodes) to bound the live
+    // values; per-axis values of row (i, j) are picked by select.
 #pragma unroll 1
     for (int ij = 0; ij < 9; ++ij) {
       const int i = ij / 3, jj = ij - 3 * (ij / 3);
@@ -794,57 +954,66 @@ __global__ __launch_bounds__(256) void k_adj_gather(const mpm_instance* inst,
       const T dxi = i == 0 ? dw[0][0] : (i == 1 ? dw[0][1] : dw[0][2]);
       const T wyj = jj == 0 ? w[1][0] : (jj == 1 ? w[1][1] : w[1][2]);
       const T dyj = jj == 0 ? dw[1][0] : (jj == 1 ? dw[1][1] : dw[1][2]);
-      const int nxi = i == 0 ? nx.idx[0] : (i == 1 ? nx.idx[1] : nx.idx[2]);
-      const int nyj = jj == 0 ? ny.idx[0] : (jj == 1 ? ny.idx[1] : ny.idx[2]);
-      const bool okxy = (i == 0 ? nx.ok[0] : (i == 1 ? nx.ok[1] : nx.ok[2])) &&
-                        (jj == 0 ? ny.ok[0] : (jj == 1 ? ny.ok[1] : ny.ok[2]));
       const T Ti = T(i), Tj = T(jj);
-      const T wij = wxi * wyj;
-      const T d0 = (Ti - fx[0]) * I.dx, d1 = (Tj - fx[1]) * I.dx;
-      T u1[3], u2[3];
-      for (int a = 0; a < 3; ++a) {
-        u1[a] = q1[a] + Ti * P1(a, 0) + Tj * P1(a, 1);
-        u2[a] = q2[a] + Ti * Ap(a, 0) + Tj * Ap(a, 1);
+      const T wxy = wxi * wyj;
+      V4<T> ga[3];
+      for (int k = 0; k < 3; ++k) ga[k] = adj_nodes(i, jj, k);
+      for (int c = 0; c < 3; ++c) {
+        const T m0 = c == 0 ? ga[0].y : (c == 1 ? ga[0].z : ga[0].w);
+        const T m1 = c == 0 ? ga[1].y : (c == 1 ? ga[1].z : ga[1].w);
+        const T m2 = c == 0 ? ga[2].y : (c == 1 ? ga[2].z : ga[2].w);
+        const T r0 = wz0 * m0 + wz1 * m1 + wz2 * m2;
+        const T r1 = wz1 * m1 + wz2k * m2;
+        vb[c] += wxy * r0;
+        Mo(c, 0) += (Ti * wxy) * r0;
+        Mo(c, 1) += (Tj * wxy) * r0;
+        Mo(c, 2) += wxy * r1;
       }
-      const int64_t row = I.g_off + ((int64_t)nxi * I.ny + nyj) * I.nz;
+      if (advect) {
+        T u1[3], u2[3];
+        for (int c = 0; c < 3; ++c) {
+          u1[c] = q1[c] + Ti * P1(c, 0) + Tj * P1(c, 1);
+          u2[c] = q2[c] + Ti * Ap(c, 0) + Tj * Ap(c, 1);
+        }
+        T A0 = T(0), A1 = T(0);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const int64_t gi = row + nz.idx[k];
-        const bool ok = okxy && nz.ok[k];
-        const V4<T> ga = ld4(gadj + 4 * gi);
-        const T wt = wij * w[2][k];
-        const T d[3] = {d0, d1, (T(k) - fx[2]) * I.dx};
-        const T mb[3] = {ok ? ga.y : T(0), ok ? ga.z : T(0), ok ? ga.w : T(0)};
-        for (int a = 0; a < 3; ++a) {
-          vb[a] += wt * mb[a];
-          for (int b = 0; b < 3; ++b) MD(a, b) += wt * (mb[a] * d[b]);
+        for (int k = 0; k < 3; ++k) {
+          const V4<T> gv = vel_nodes(i, jj, k);
+          const T vi[3] = {gv.y, gv.z, gv.w};
+          const T mb[3] = {ga[k].y, ga[k].z, ga[k].w};
+          T alpha = ga[k].x * I.pmass;
+          for (int c = 0; c < 3; ++c)
+            alpha += vi[c] * (u1[c] + T(k) * P1(c, 2)) + mb[c] * (u2[c] + T(k) * Ap(c, 2));
+          A0 += w[2][k] * alpha;
+          A1 += dw[2][k] * alpha;
+          const T wt = wxy * w[2][k];
+          Sv[0] += wt * gv.y;
+          Sv[1] += wt * gv.z;
+          Sv[2] += wt * gv.w;
         }
-        if (advect) {
-          const V4<T> gv = ld4(grid + 4 * gi);
-          const T vi[3] = {ok ? gv.y : T(0), ok ? gv.z : T(0), ok ? gv.w : T(0)};
-          const T gmb = ok ? ga.x : T(0);
-          const T gw[3] = {dxi * wyj * w[2][k] * I.inv_dx, wxi * dyj * w[2][k] * I.inv_dx,
-                           wij * dw[2][k] * I.inv_dx};
-          T alpha = gmb * I.pmass;
-          for (int a = 0; a < 3; ++a)
-            alpha += vi[a] * (u1[a] + T(k) * P1(a, 2)) + mb[a] * (u2[a] + T(k) * Ap(a, 2));
-          for (int a = 0; a < 3; ++a) {
-            xb[a] += alpha * gw[a];
-            Sv[a] += wt * vi[a];
-          }
-        }
+        xb[0] += (dxi * wyj * I.inv_dx) * A0;
+        xb[1] += (wxi * dyj * I.inv_dx) * A0;
+        xb[2] += (wxy * I.inv_dx) * A1;
       }
     }
+  };
+  if (valid) {
+    if (bx.staged)
+      gather(box_nodes(lg, bx, base), box_nodes(lv, bx, base));
+    else
+      gather(grid_nodes(gadj, I, base), grid_nodes(grid, I, base));
   }
   if (advect) {
     // x_bar += xn_b - (4 inv_dx^2 C_b')^T (sum w v_i) - A^T (sum w mb_i), A = A'/dx
     T xnb[3];
     ld3(bar_out, n_slots, p, RX, xnb);
-    for (int a = 0; a < 3; ++a)
-      xb[a] += xnb[a] - ((P1(0, a) * Sv[0] + P1(1, a) * Sv[1] + P1(2, a) * Sv[2]) +
-                         (Ap(0, a) * vb[0] + Ap(1, a) * vb[1] + Ap(2, a) * vb[2])) * I.inv_dx;
+    for (int c = 0; c < 3; ++c)
+      xb[c] += xnb[c] - ((P1(0, c) * Sv[0] + P1(1, c) * Sv[1] + P1(2, c) * Sv[2]) +
+                         (Ap(0, c) * vb[0] + Ap(1, c) * vb[1] + Ap(2, c) * vb[2])) * I.inv_dx;
   }
-  for (int a = 0; a < 3; ++a) vb[a] *= I.pmass;
+  for (int c = 0; c < 3; ++c)
+    for (int e = 0; e < 3; ++e) MD(c, e) = I.dx * (Mo(c, e) - vb[c] * fx[e]);
+  for (int c = 0; c < 3; ++c) vb[c] *= I.pmass;
   st3(bar_in, n_slots, p, RX, xb);
   st3(bar_in, n_slots, p, RV, vb);
   st9(md, n_slots, j, 0, MD);
@@ -887,10 +1056,9 @@ __global__ __launch_bounds__(256) void k_adj_vjp(const mpm_instance* inst, const
     {
       const M3<T> Cn = ld9(dyn_out, n_slots, p, RC);
       const M3<T> Fnb = ld9(bar_out, n_slots, p, RF);
-      M3<T> IC = scaled(Cn, I.dt);
-      IC.a[0] += T(1); IC.a[4] += T(1); IC.a[8] += T(1);
-      const M3<T> Fb = mul_tn(IC, Fnb);
-      for (int k = 0; k < 9; ++k) Fpb.a[k] += Fb.a[k];
+      // (I + dt C_new)^T Fn_b, formed as Fn_b + dt C_new^T Fn_b
+      const M3<T> CtF = mul_tn(Cn, Fnb);
+      for (int k = 0; k < 9; ++k) Fpb.a[k] += Fnb.a[k] + I.dt * CtF.a[k];
       st9(bar_in, n_slots, p, RF, Fpb);
     }
     const M3<T> GiT = inv_t(Fg);
@@ -1124,9 +1292,11 @@ void launch_tile_fwd(const mpm_batch* b, const SortView<const int32_t>& v, const
   k_p2g_tile<T, TX, TY, TZ, true><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
       b->inst, b->tile_inst, v.active_unit, v.n_active, v.start, payload, (T*)w.partial);
   }
+  {
   TimedLaunch tl(MPM_K_GRID_FWD, st);
   k_grid_fwd<T, TX, TY, TZ><<<grid1(b->n_nodes, 256), 256, 0, st>>>(
       b->inst, b->node_inst, b->n_nodes, v.unit_rank, (const T*)w.partial, grid);
+  }
 }
 
 template <typename T, int TX, int TY, int TZ>
@@ -1138,10 +1308,12 @@ void launch_tile_adj(const mpm_batch* b, const SortView<const int32_t>& v, const
       b->inst, b->tile_inst, v.active_unit, v.n_active, v.start, (const T*)w.payload,
       (T*)w.partial);
   }
+  {
   TimedLaunch tl(MPM_K_GRID_ADJ, st);
   k_grid_adj<T, TX, TY, TZ><<<grid1(b->n_nodes, 256), 256, 0, st>>>(
       b->inst, b->node_inst, b->n_nodes, v.unit_rank, (const T*)w.partial, grid, (T*)w.gadj,
       advect);
+  }
 }
 
 template <typename T>
@@ -1219,11 +1391,10 @@ int step_bwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
                                                       bar_out, rank, (T*)w.payload, advect);
   }
   dispatch_adj<T>(tile_index(b), b, sv, w, grid, advect, st);
-  const int32_t* sorted = sv.sorted;
   {
   TimedLaunch tl(MPM_K_P2G_ADJ, st);
-  k_adj_gather<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(
-      b->inst, b->slot_inst, b->n_slots, sorted, payload, (const T*)w.payload, grid,
+  k_adj_gather<T><<<dim3((unsigned)(b->n_slots / kChunk)), kChunk, 0, st>>>(
+      b->inst, b->slot_inst, b->n_slots, sv.sorted, payload, (const T*)w.payload, grid,
       (const T*)w.gadj, bar_out, bar_in, (T*)w.md, advect);
   }
   TimedLaunch tl(MPM_K_ADJ_VJP, st);

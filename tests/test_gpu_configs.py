@@ -70,9 +70,9 @@ def test_free_swell_cube_fp64_matches_oracle():
 
 def test_free_swell_rate_gradient_fd_and_fp32():
     """d(mean |x - x_target|^2)/d(rate): FP64 adjoint vs central FD (rel 1e-6),
-    FP32 adjoint vs FP64 within 5e-5 (measured 2.3e-5: the FP32 elastic strain
-    Fe - I ~ 1e-3 is resolved to ~1e-4 relative from O(1) F and Fg, and this
-    loss is a small difference of trajectories)."""
+    FP32 adjoint vs FP64 within 1e-5.  This loss is a small difference of
+    trajectories, so it sees any systematic FP32 forward error directly (the
+    F update is formed as F + dt C F for that reason)."""
     from morphompm_amd.growth import position_misfit
     from morphompm_amd import autograd as AG
     from morphompm_amd.scenes import free_swell_cube
@@ -89,7 +89,7 @@ def test_free_swell_rate_gradient_fd_and_fp32():
     fd = (Lp - Lm) / (2 * h)
     assert abs(g64 - fd) <= 1e-6 * abs(fd), (g64, fd)
     L32, g32 = _rate_loss_grad(sc, 0.09, torch.float32, loss)
-    assert abs(g32 - g64) <= 5e-5 * abs(g64), (g32, g64)
+    assert abs(g32 - g64) <= 1e-5 * abs(g64), (g32, g64)
 
 
 def test_free_swell_det_relaxes_toward_g_cubed():
