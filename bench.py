@@ -58,8 +58,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
-    ap.add_argument("--tile", default="4,8,8")
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--tile", default="auto", help='P2G tile "x,y,z" or "auto" (autotuner)')
+    ap.add_argument("--block", default="auto", help='particle-kernel block size or "auto"')
     ap.add_argument("--pmc-pass", action="store_true",
                     help="for rocprofv3 --pmc: a device copy of --copy-mb MiB (counter "
                          "calibration), then one eager forward + adjoint iteration; no timing")
@@ -136,7 +136,8 @@ def main():
     import ctypes
 
     _lib.require_gpu()
-    tile = tuple(int(t) for t in a.tile.split(","))
+    tile = "auto" if a.tile == "auto" else tuple(int(t) for t in a.tile.split(","))
+    block = "auto" if a.block == "auto" else int(a.block)
     scenes = batch_instances(count=a.instances, n_steps=a.sim_steps)
     for s in scenes:
         s.checkpoint_every = a.checkpoint_every
@@ -146,12 +147,16 @@ def main():
           f"K={a.checkpoint_every}", file=sys.stderr, flush=True)
 
     if a.pmc_pass:
+        # counters are read per kernel name: no tuning dispatches in this run
+        from morphompm_amd.engine import DEFAULT_TILE
+        tile = DEFAULT_TILE if tile == "auto" else tile
+        block = 256 if block == "auto" else block
         src = torch.ones(a.copy_mb << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        bi = BatchedInverse(scenes, dtype=torch.float32, tile=tile, block=a.block,
+        bi = BatchedInverse(scenes, dtype=torch.float32, tile=tile, block=block,
                             use_graphs=False)
         bi.set_theta(rates)
         bi.loss_and_grad()
@@ -159,7 +164,16 @@ def main():
         print(json.dumps({"pmc_pass": True, "copy_bytes": a.copy_mb << 20,
                           "particles": n_total}))
         return
-    bi = BatchedInverse(scenes, dtype=torch.float32, tile=tile, block=a.block, use_graphs=True)
+    t_tune = time.perf_counter()
+    bi = BatchedInverse(scenes, dtype=torch.float32, tile=tile, block=block, use_graphs=True)
+    tune = None
+    if bi.tune is not None:
+        tune = {"shape_class": bi.tune.key, "cached": bi.tune.cached,
+                "ms_per_fwd_rev_step": bi.tune.ms,
+                "setup_s": round(time.perf_counter() - t_tune, 2)}
+        print(f"[bench] autotuner: tile {bi.plan.tile} block {bi.plan.block} "
+              f"({'cached' if bi.tune.cached else 'tuned'})", file=sys.stderr, flush=True)
+    tile, block = bi.plan.tile, bi.plan.block
     # targets: the loss at the true rates (the observed shapes)
     bi.set_theta(rates)
     target, _ = bi.loss_and_grad()
@@ -259,7 +273,7 @@ def main():
         for i in idx:
             s = scenes[i]
             p = BatchPlan([InstanceSpec.from_cfg(s.n, s.cfg, s.model, s.pts)], tile=tile,
-                          block=a.block)
+                          block=block)
             tr = Trajectory(p, torch.float32, s.n_steps, True, a.checkpoint_every, s.sched)
             dyn0 = p.pack_dyn([(s.pts, np.zeros((s.n, 3)), np.zeros((s.n, 3, 3)),
                                 np.repeat(np.eye(3)[None], s.n, 0))], torch.float32)
@@ -288,7 +302,7 @@ def main():
         F32 = bi.final_state()[ROW_F:ROW_F + 9].double().clone()
         del bi
         torch.cuda.empty_cache()
-        b64 = BatchedInverse(scenes, dtype=torch.float64, tile=tile, block=a.block,
+        b64 = BatchedInverse(scenes, dtype=torch.float64, tile=tile, block=block,
                              use_graphs=False)
         b64.set_theta(theta)
         L64, g64 = (t.cpu().numpy() for t in b64.loss_and_grad())
@@ -314,7 +328,8 @@ def main():
                                   "forward + checkpointed adjoint of every instance)",
                       "instances": len(scenes), "particles": n_total,
                       "sim_steps": a.sim_steps, "checkpoint_every": a.checkpoint_every,
-                      "tile": list(tile), "block": a.block, "hip_graphs": True},
+                      "tile": list(tile), "block": block, "hip_graphs": True},
+           "autotune": tune,
            "roofline": roof, "whole_step_roofline": whole, "kernels": ktab,
            "cpu_baseline": cpu,
            "plain_gpu": plain,

@@ -1,0 +1,150 @@
+"""Launch-shape autotuner for a batch.
+
+Two launch parameters are free: the P2G tile shape (cells per P2G block; the
+sort bins particles by these tiles) and the block size of the particle
+kernels (particles per block).  ``autotune`` builds a ``BatchPlan`` for every
+candidate, runs a few forward steps of the batch's own particles from rest
+under their growth tensors, then times forward + reverse steps with HIP events
+and keeps the fastest.  Results are cached per *shape class* -- dtype, model
+mix, advection, and log2 buckets of the particle count, the instance count and
+the mean grid size -- in a JSON file, so a workload is tuned once:
+
+    tile, block = autotune(specs, pts, dtype, advect, G_list).choice
+
+The cache file is ``$MORPHOMPM_AMD_TUNE_CACHE`` if set, else
+``~/.cache/morphompm_amd/autotune.json``; ``cache=False`` skips it.
+"""
+import json
+import math
+import os
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .engine import BatchPlan
+
+CANDIDATE_BLOCKS = (64, 128, 256)
+
+
+def cache_path():
+    p = os.environ.get("MORPHOMPM_AMD_TUNE_CACHE")
+    if p:
+        return p
+    return os.path.join(os.path.expanduser("~"), ".cache", "morphompm_amd", "autotune.json")
+
+
+def shape_class(specs, dtype, advect):
+    """Cache key of a batch: what the best launch shape depends on."""
+    n = sum(s.n for s in specs)
+    nodes = np.mean([np.prod(s.shape) for s in specs])
+    kinds = sorted({int(s.kind) for s in specs})
+    dt = "f64" if dtype == torch.float64 else "f32"
+    b = lambda v: int(round(math.log2(max(v, 1))))
+    return (f"{dt}-m{''.join(map(str, kinds))}-a{int(bool(advect))}"
+            f"-n{b(n)}-i{b(len(specs))}-g{b(nodes)}")
+
+
+def load_cache(path):
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d if isinstance(d, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def store_cache(path, key, entry):
+    """Adds one entry; never raises (a read-only home just skips caching)."""
+    try:
+        d = load_cache(path)
+        d[key] = entry
+        os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+        tmp = f"{path}.{os.getpid()}.tmp"
+        with open(tmp, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+        return True
+    except OSError:
+        return False
+
+
+@dataclass
+class TuneResult:
+    tile: tuple
+    block: int
+    ms: dict = field(default_factory=dict)   # "tx,ty,tz/block" -> ms per fwd+rev step
+    key: str = ""
+    cached: bool = False
+
+    @property
+    def choice(self):
+        return self.tile, self.block
+
+
+def _label(tile, block):
+    return f"{tile[0]},{tile[1]},{tile[2]}/{block}"
+
+
+def _time_candidate(specs, pts_list, G_list, dtype, advect, tile, block, warm, reps):
+    plan = BatchPlan(specs, tile=tile, block=block)
+    perm = plan.cell_order(pts_list)
+    parts = []
+    for s, p in zip(specs, pts_list):
+        k = s.n
+        parts.append((p, np.zeros((k, 3)), np.zeros((k, 3, 3)), np.repeat(np.eye(3)[None], k, 0)))
+    a = plan.pack_dyn(parts, dtype, perm)
+    G = plan.pack_mat(G_list, dtype, perm)
+    b = plan.new_dyn(dtype)
+    grid, sb, pl = plan.new_grid(dtype), plan.new_sort(), plan.new_payload(dtype)
+    for _ in range(warm):                      # evolve the state off rest
+        plan.step_fwd(a, G, None, b, grid, sb, pl, advect)
+        a, b = b, a
+    bar_out = plan.new_dyn(dtype)
+    bar_out[15:24] = plan.live.to(dtype)       # a unit F cotangent
+    bar_in, G_bar = plan.new_dyn(dtype), plan.new_mat(dtype)
+    theta = torch.zeros(plan.n_slots // 256, 2, dtype=torch.float64, device=plan.device)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def one():
+        plan.step_fwd(a, G, None, b, grid, sb, pl, advect)
+        plan.step_bwd(a, G, None, b, grid, sb, pl, bar_out, bar_in, G_bar, theta, advect)
+    one()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        one()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def autotune(specs, pts_list, dtype=torch.float32, advect=True, G_list=None, tiles=None,
+             blocks=CANDIDATE_BLOCKS, warm=3, reps=3, cache=True, path=None):
+    """Best (tile, block) for this batch; see the module docstring."""
+    _lib.require_gpu()
+    key = shape_class(specs, dtype, advect)
+    path = path or cache_path()
+    built = [tuple(t) for t in _lib.tile_shapes()]
+    tiles = [tuple(t) for t in (tiles or built)]
+    if cache:
+        hit = load_cache(path).get(key)
+        if hit and tuple(hit["tile"]) in built and hit["block"] in blocks:
+            return TuneResult(tuple(hit["tile"]), int(hit["block"]), hit.get("ms", {}), key, True)
+    if G_list is None:
+        G_list = [np.repeat((1.05 * np.eye(3))[None], s.n, 0) for s in specs]
+    ms = {}
+    t0 = time.perf_counter()
+    for tile in tiles:
+        for block in blocks:
+            ms[_label(tile, block)] = _time_candidate(specs, pts_list, G_list, dtype, advect, tile,
+                                                      block, warm, reps)
+    best = min(ms, key=ms.get)
+    tl, bl = best.split("/")
+    res = TuneResult(tuple(int(v) for v in tl.split(",")), int(bl), ms, key, False)
+    if cache:
+        store_cache(path, key, {"tile": list(res.tile), "block": res.block, "ms": ms,
+                                "tuned_s": round(time.perf_counter() - t0, 3)})
+    return res

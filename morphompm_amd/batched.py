@@ -39,6 +39,9 @@ def cofactor_rows(F):
 class BatchedInverse:
     def __init__(self, scenes, dtype=torch.float32, tile=DEFAULT_TILE, block=256,
                  use_graphs=True, checkpoint_every=None):
+        """``tile`` / ``block``: the P2G tile shape and particle-kernel block
+        size, or "auto" to pick them with the autotuner (cached per shape
+        class)."""
         self.scenes = scenes
         steps = {s.n_steps for s in scenes}
         if len(steps) != 1:
@@ -50,6 +53,14 @@ class BatchedInverse:
         self.advect = adv.pop()
         self.dtype = dtype
         specs = [InstanceSpec.from_cfg(s.n, s.cfg, s.model, s.pts, center=True) for s in scenes]
+        self.tune = None
+        if tile == "auto" or block == "auto":
+            from .autotune import autotune
+            G_list = [np.eye(3) + s.extra.get("rate", 0.05) * s.dG for s in scenes]
+            self.tune = autotune(specs, [s.pts for s in scenes], dtype, self.advect, G_list,
+                                 tiles=None if tile == "auto" else [tile],
+                                 blocks=(64, 128, 256) if block == "auto" else (block,))
+            tile, block = self.tune.choice
         self.plan = p = BatchPlan(specs, tile=tile, block=block)
         K = checkpoint_every or scenes[0].checkpoint_every
         sched = np.stack([s.sched if s.sched is not None else np.ones(self.n_steps)
