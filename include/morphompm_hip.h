@@ -25,7 +25,7 @@
  *   compact index unit_rank[u] (-1 when empty), active_unit inverts it, and
  *   bin_start covers n_active * TB bins.  Each instance's sorted range is its
  *   own slot range; padding slots sort into the instance's dead unit.
- * P2G payload for one step: payload [n_slots][16] in sorted order, per
+ * P2G payload for one step: payload [n_slots][16] in slot order, per
  *   particle (fx[3], q[3], A'[9], cell) -- the forward step's momentum map
  *   m_i += w pmass, mom_i += w (q + A' o) for stencil offset o, and the packed
  *   cell ((base+2) per axis in 10 bits, all ones off the grid).  Grids are
@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 2
+#define MPM_ABI_VERSION 3
 #define MPM_SLOT_ALIGN 256 /* particle and node ranges of an instance start at multiples of this */
 
 enum { MPM_OK = 0, MPM_EINVAL = -1, MPM_ELAUNCH = -2, MPM_ENOTSUP = -3 };
@@ -145,6 +145,26 @@ int32_t mpm_step_fwd_f32(const mpm_batch* b, const float* dyn_in, const float* F
 int32_t mpm_step_fwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
                          const double* sched, double* dyn_out, double* grid, int32_t* sortbuf,
                          double* payload, int32_t advect, void* ws, void* stream);
+
+/* The same step for a chain of steps on one workspace and stream, with the
+ * next step's P2G payload produced by this step's G2P (the per-particle prep
+ * pass and the sort's position pass are then done once per chain, not once
+ * per step).  ready = 1: `payload` and the sort keys in `ws` were written by
+ * the previous chained call (its payload_next), which must be the last call
+ * on this workspace; ready = 0: they are computed from dyn_in first.
+ * payload_next (or NULL at the end of a chain) receives the next step's
+ * payload, with the next step's growth factors sched_next.  Results are bit
+ * for bit those of mpm_step_fwd.  No reference counterpart (a fusion of
+ * consecutive transfer.step calls, python/morphompm/integrate.py:15-21). */
+int32_t mpm_step_fwd_chain_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
+                               const float* sched, const float* sched_next, float* dyn_out,
+                               float* grid, int32_t* sortbuf, float* payload, float* payload_next,
+                               int32_t advect, int32_t ready, void* ws, void* stream);
+int32_t mpm_step_fwd_chain_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
+                               const double* sched, const double* sched_next, double* dyn_out,
+                               double* grid, int32_t* sortbuf, double* payload,
+                               double* payload_next, int32_t advect, int32_t ready, void* ws,
+                               void* stream);
 
 /* Reverse of one step.  Replaces transfer.step_vjp
  * (python/morphompm/transfer.py:82-158).  Inputs are the forward step's

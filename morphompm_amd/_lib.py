@@ -11,7 +11,7 @@ import torch
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libmorphompm_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 SLOT_ALIGN = 256
 
 NEO_HOOKEAN, HENCKY, HERSCHEL_BULKLEY = 0, 1, 2
@@ -71,6 +71,9 @@ def lib():
         f = getattr(L, f"mpm_step_fwd_{suf}")
         f.restype = _I32
         f.argtypes = [ctypes.POINTER(MpmBatch), _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P]
+        f = getattr(L, f"mpm_step_fwd_chain_{suf}")
+        f.restype = _I32
+        f.argtypes = [ctypes.POINTER(MpmBatch)] + [_P] * 9 + [_I32, _I32, _P, _P]
         f = getattr(L, f"mpm_step_bwd_{suf}")
         f.restype = _I32
         f.argtypes = [ctypes.POINTER(MpmBatch)] + [_P] * 11 + [_I32, _P, _P]

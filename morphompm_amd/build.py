@@ -12,6 +12,10 @@ OUT = os.path.join(HERE, "_lib")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
+             # multiply-adds fused per source expression only, so a kernel's
+             # rounding does not depend on which template instance it is in
+             # (a batch of mixed models gives each instance the bits it gets alone)
+             "-ffp-contract=on",
              # no packed-FP32 instruction selection
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

@@ -199,39 +199,8 @@ __device__ __forceinline__ void bin_unit(const mpm_instance& I, int bx, int by, 
   local = ((bx - itx * TX) * TY + (by - ity * TY)) * TZ + (bz - itz * TZ);
 }
 
-// Unit and place of a live particle; false when no stencil node is in the grid.
-template <typename T>
-__device__ __forceinline__ bool particle_unit(const mpm_instance& I, const T* dyn, int64_t n_slots,
-                                              int64_t p, int TX, int TY, int TZ, int64_t& unit,
-                                              int& local) {
-  T x[3], fx[3];
-  int base[3];
-  ld3(dyn, n_slots, p, RX, x);
-  stencil_base(x, (T)(1.0 / I.dx), I.origin, base, fx);
-  const int bx = base[0] + 2, by = base[1] + 2, bz = base[2] + 2;
-  if (bx < 0 || bx >= I.nx + 2 || by < 0 || by >= I.ny + 2 || bz < 0 || bz >= I.nz + 2)
-    return false;
-  bin_unit(I, bx, by, bz, TX, TY, TZ, unit, local);
-  return true;
-}
-
 __device__ __forceinline__ int64_t dead_unit(const mpm_instance& I) {
   return I.t_off + (int64_t)I.tx * I.ty * I.tz;
-}
-
-// Flags the units that hold particles (plain stores of 1: order-free).
-template <typename T>
-__global__ __launch_bounds__(256) void k_unit_mark(const mpm_instance* inst,
-                                                   const int32_t* slot_inst, int64_t n_slots,
-                                                   const T* dyn, int TX, int TY, int TZ,
-                                                   int32_t* unit_flag) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_slots) return;
-  const mpm_instance& I = inst[block_inst(slot_inst)];
-  if (p - I.p_off >= I.n) return;
-  int64_t unit;
-  int local;
-  if (particle_unit(I, dyn, n_slots, p, TX, TY, TZ, unit, local)) unit_flag[unit] = 1;
 }
 
 // Units are ranked in two passes over 256-unit blocks: k_unit_count counts
@@ -295,14 +264,17 @@ __global__ __launch_bounds__(256) void k_unit_rank(const int32_t* unit_inst, int
     *n_active = base + wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-template <typename T>
+// Turns the unit-level keys (unit * TB + place) into compact bin keys of
+// this step's occupied units, in place, and counts the bins.
+// A key outside the flagged units (only possible when a chained step is
+// told its keys are ready and they are not) goes to the dead bin: wrong, but
+// every index stays in bounds.
 __global__ __launch_bounds__(256) void k_sort_key(const mpm_instance* inst, const int32_t* slot_inst,
-                                                  int64_t n_slots, const T* dyn, int TX, int TY,
-                                                  int TZ, const int32_t* unit_rank, int32_t* key,
+                                                  int64_t n_slots, int TB, int64_t n_units,
+                                                  const int32_t* unit_rank, int32_t* key,
                                                   int32_t* slot, int32_t* count) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const mpm_instance& I = inst[block_inst(slot_inst)];
-  const int TB = TX * TY * TZ;
   // The dead unit's bin 0 holds live particles with no in-grid node and its
   // bin 1 the padding slots, so each instance's sorted range is exactly its
   // own slot range [p_off, p_off + padded count): padding slots take the tail
@@ -317,11 +289,9 @@ __global__ __launch_bounds__(256) void k_sort_key(const mpm_instance* inst, cons
       atomicAdd(&count[dead + 1], (int32_t)((padded > 0 ? padded : kChunk) - I.n));
     }
     if (local < I.n) {
-      int64_t unit;
-      int l;
-      k = particle_unit(I, dyn, n_slots, p, TX, TY, TZ, unit, l)
-              ? (int64_t)unit_rank[unit] * TB + l
-              : dead;
+      const int32_t raw = key[p];
+      const int32_t ur = (raw >= 0 && raw / TB < n_units) ? unit_rank[raw / TB] : -1;
+      k = ur >= 0 ? (int64_t)ur * TB + raw % TB : dead;
     }
   }
   // One atomic per run of equal keys in adjacent lanes (particles arrive
@@ -452,28 +422,34 @@ __device__ __forceinline__ void store_payload(T* r, const T fx[3], const T q[3],
   st4(r + 12, A.a[6], A.a[7], A.a[8], cell_as(T(0), cell));
 }
 
+// Sort key of a particle with stencil base `base`, the input of the next
+// sort: unit * TB + place in the unit, or dead unit * TB when no stencil node
+// is in the grid.  Flags the unit as occupied.
+__device__ __forceinline__ int32_t cell_key(const mpm_instance& I, const int base[3], int TX,
+                                            int TY, int TZ, int32_t* unit_flag) {
+  const int TB = TX * TY * TZ;
+  const int bx = base[0] + 2, by = base[1] + 2, bz = base[2] + 2;
+  if (bx < 0 || bx >= I.nx + 2 || by < 0 || by >= I.ny + 2 || bz < 0 || bz >= I.nz + 2)
+    return (int32_t)(dead_unit(I) * TB);
+  int64_t unit;
+  int local;
+  bin_unit(I, bx, by, bz, TX, TY, TZ, unit, local);
+  unit_flag[unit] = 1;
+  return (int32_t)(unit * TB + local);
+}
+
 // ---------------------------------------------------------------------------
 // forward prep: stress, grown volume, APIC payload (transfer.py:43-45,53)
 //   m_i += w pmass;  mom_i += w (P + A dpos),  P = pmass v,  A = pmass C + dt c tau,
 //   c = -4 inv_dx^2 vol0 det Fg,  dpos = (o - fx) dx  ->  A dpos = A' (o - fx),
-//   A' = A dx;  stored: fx, Q = P - A' fx, A'.
+//   A' = A dx;  stored: fx, Q = P - A' fx, A'.  Payload records are in slot
+//   order; P2G reads them through the sort.
 // ---------------------------------------------------------------------------
-template <typename T, int MODEL>
-__global__ __launch_bounds__(256) void k_prep_fwd(const mpm_instance* inst, const int32_t* slot_inst, int64_t n_slots,
-                           const T* dyn, const T* Fgp, const T* sched, const int32_t* rank,
-                           T* payload) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_slots) return;
-  const int ii = block_inst(slot_inst);
-  const Inst<T> I = load_inst<T, MODEL>(inst, ii);
-  if (p - I.p_off >= I.n) return;
-  T x[3], v[3], fx[3];
-  int base[3];
-  ld3(dyn, n_slots, p, RX, x);
-  ld3(dyn, n_slots, p, RV, v);
-  const M3<T> C = ld9(dyn, n_slots, p, RC);
-  const M3<T> F = ld9(dyn, n_slots, p, RF);
-  const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), sched ? sched[ii] : T(1));
+template <typename T>
+__device__ __forceinline__ void make_payload(const Inst<T>& I, const T x[3], const T v[3],
+                                             const M3<T>& C, const M3<T>& F, const M3<T>& Fg,
+                                             T* rec, int base[3]) {
+  T fx[3];
   stencil_base(x, I, base, fx);
   const M3<T> tau = kirchhoff(I.mp, F, Fg, C);
   const T c = -I.k4 * I.vol0 * det(Fg);
@@ -482,8 +458,48 @@ __global__ __launch_bounds__(256) void k_prep_fwd(const mpm_instance* inst, cons
   T q[3];
   for (int a = 0; a < 3; ++a)
     q[a] = I.pmass * v[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
-  store_payload(payload + (int64_t)rank[p] * kRecord, fx, q, Ap,
-                pack_cell(base, I.nx, I.ny, I.nz));
+  store_payload(rec, fx, q, Ap, pack_cell(base, I.nx, I.ny, I.nz));
+}
+
+// Payload, sort key and unit flag of a step from its input state.
+template <typename T, int MODEL>
+__global__ __launch_bounds__(256) void k_prep_fwd(const mpm_instance* inst,
+                                                  const int32_t* slot_inst, int64_t n_slots,
+                                                  const T* dyn, const T* Fgp, const T* sched,
+                                                  int TX, int TY, int TZ, T* payload, int32_t* key,
+                                                  int32_t* unit_flag) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_slots) return;
+  const int ii = block_inst(slot_inst);
+  const Inst<T> I = load_inst<T, MODEL>(inst, ii);
+  if (p - I.p_off >= I.n) return;
+  T x[3], v[3];
+  ld3(dyn, n_slots, p, RX, x);
+  ld3(dyn, n_slots, p, RV, v);
+  const M3<T> C = ld9(dyn, n_slots, p, RC);
+  const M3<T> F = ld9(dyn, n_slots, p, RF);
+  const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), sched ? sched[ii] : T(1));
+  int base[3];
+  make_payload(I, x, v, C, F, Fg, payload + p * kRecord, base);
+  key[p] = cell_key(inst[ii], base, TX, TY, TZ, unit_flag);
+}
+
+// Sort keys and unit flags from positions only (mpm_sort_by_cell).
+template <typename T>
+__global__ __launch_bounds__(256) void k_cell_keys(const mpm_instance* inst,
+                                                   const int32_t* slot_inst, int64_t n_slots,
+                                                   const T* dyn, int TX, int TY, int TZ,
+                                                   int32_t* key, int32_t* unit_flag) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_slots) return;
+  const int ii = block_inst(slot_inst);
+  const Inst<T> I = load_inst<T>(inst, ii);
+  if (p - I.p_off >= I.n) return;
+  T x[3], fx[3];
+  int base[3];
+  ld3(dyn, n_slots, p, RX, x);
+  stencil_base(x, I, base, fx);
+  key[p] = cell_key(inst[ii], base, TX, TY, TZ, unit_flag);
 }
 
 // ---------------------------------------------------------------------------
@@ -493,12 +509,15 @@ __global__ __launch_bounds__(256) void k_prep_fwd(const mpm_instance* inst, cons
 // n_active and dead units return at once).  partial: [active unit][4][TN] --
 // row 0 sum of weights (MASS), rows 1-3 sum of w (Q + A' o).
 // ---------------------------------------------------------------------------
-template <typename T, int TX, int TY, int TZ, bool MASS>
+// INDIRECT: payload records in slot order, read through sorted[j]; else in
+// sorted order.
+template <typename T, int TX, int TY, int TZ, bool MASS, bool INDIRECT>
 __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* inst,
                                                         const int32_t* unit_inst,
                                                         const int32_t* active_unit,
                                                         const int32_t* n_active,
                                                         const int32_t* bin_start,
+                                                        const int32_t* sorted,
                                                         const T* payload, T* partial) {
   constexpr int NY = TY + 2, NZ = TZ + 2, TN = (TX + 2) * NY * NZ;
   constexpr int NT = TX * TY * TZ;
@@ -522,7 +541,7 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
   }
   const int32_t s0 = bin_start[key0 + tid], s1 = bin_start[key0 + tid + 1];
   for (int32_t j = s0; j < s1; ++j) {
-    const T* r = payload + (int64_t)j * kRecord;
+    const T* r = payload + (int64_t)(INDIRECT ? sorted[j] : j) * kRecord;
     const V4<T> r0 = ld4(r), r1 = ld4(r + 4), r2 = ld4(r + 8), r3 = ld4(r + 12);
     // fx = r0.xyz, Q = (r0.w, r1.x, r1.y), A' row-major = r1.zw r2.xyzw r3.xyz
     const T q0 = r0.w, q1 = r1.x, q2 = r1.y;
@@ -746,14 +765,20 @@ __device__ __forceinline__ GridNodes<T> grid_nodes(const T* rec, const Inst<T>& 
 // ---------------------------------------------------------------------------
 // G2P (transfer.py:61-77), one thread per particle slot (coalesced state
 // reads and writes); stencil nodes come from the grid through the caches.
+// NEXT: also writes the next step's payload, sort key and unit flag from the
+// new state (what k_prep_fwd would compute from dyn_out, bit for bit), with
+// the next step's growth factor sched_next.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int MODEL, bool NEXT>
 __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int32_t* slot_inst,
                                              int64_t n_slots, const T* dyn_in, const T* grid,
-                                             T* dyn_out, int advect) {
+                                             T* dyn_out, int advect, const T* Fgp,
+                                             const T* sched_next, int TX, int TY, int TZ,
+                                             T* payload_next, int32_t* key, int32_t* unit_flag) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_slots) return;
-  const Inst<T> I = load_inst<T>(inst, block_inst(slot_inst));
+  const int ii = block_inst(slot_inst);
+  const Inst<T> I = load_inst<T, MODEL>(inst, ii);
   if (p - I.p_off >= I.n) return;
   T x[3], fx[3];
   int base[3];
@@ -811,6 +836,12 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
   st3(dyn_out, n_slots, p, RV, nv);
   st9(dyn_out, n_slots, p, RC, nC);
   st9(dyn_out, n_slots, p, RF, Fn);
+  if (NEXT) {
+    const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), sched_next ? sched_next[ii] : T(1));
+    int nbase[3];
+    make_payload(I, xn, nv, nC, Fn, Fg, payload_next + p * kRecord, nbase);
+    key[p] = cell_key(inst[ii], nbase, TX, TY, TZ, unit_flag);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -880,8 +911,8 @@ __global__ __launch_bounds__(256) void k_grid_adj(const mpm_instance* inst, cons
 // reverse step, part 3a (transfer.py:100-119 x_bar terms, :129-147): the
 // P2G-adjoint gather, one thread per sorted slot j; each 256-slot chunk
 // stages its stencil box of the grid adjoint (and, with advection, of the
-// grid) in LDS.  Reads the forward payload pf (fx, q2, A', cell) and the
-// adjoint payload pa (fx, q1, P1) of slot j.  With dpos = (o - fx) dx for
+// grid) in LDS.  Reads the forward payload pf (fx, q2, A', cell) of particle
+// p = sorted[j] and the adjoint payload pa (fx, q1, P1) of j.  With dpos = (o - fx) dx for
 // stencil offset o, the per-node factors are affine in o:
 //   nv_b + 4 inv_dx^2 C_b' dpos = q1 + P1 o   (P1 = 4 inv_dx^2 dx C_b')
 //   pmass v + A dpos            = q2 + A' o   (A' = A dx, A = pmass C + dt c tau)
@@ -906,8 +937,8 @@ __global__ __launch_bounds__(256) void k_adj_gather(const mpm_instance* inst,
   M3<T> P1, Ap;
   uint32_t cell = kNoCell;
   if (live) {
-    const T* rf = pf + j * kRecord;
-    const T* ra = pa + j * kRecord;
+    const T* rf = pf + p * kRecord;  // forward payload: slot order
+    const T* ra = pa + j * kRecord;  // adjoint payload: sorted order
     const V4<T> f0 = ld4(rf), f1 = ld4(rf + 4), f2 = ld4(rf + 8), f3 = ld4(rf + 12);
     fx[0] = f0.x; fx[1] = f0.y; fx[2] = f0.z;
     q2[0] = f0.w; q2[1] = f1.x; q2[2] = f1.y;
@@ -1256,25 +1287,22 @@ struct TimedLaunch {
   }
 };
 
-template <typename T>
-int sort_impl(const mpm_batch* b, const T* dyn, int32_t* sortbuf, const Workspace& w,
-              hipStream_t st) {
+// The sort from the unit-level keys and unit flags a prep, G2P or key kernel
+// left in the workspace.
+int sort_from_keys(const mpm_batch* b, int32_t* sortbuf, const Workspace& w, hipStream_t st) {
   const SortView<int32_t> v(b, sortbuf);
   const int bs = b->block;
   const int TB = b->tile[0] * b->tile[1] * b->tile[2];
   const int64_t nb = scan_blocks(b->n_bins);
   const int64_t nu = (b->n_tiles + 255) / 256;
   TimedLaunch tl(MPM_K_SORT, st);
-  k_unit_mark<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn,
-                                                        b->tile[0], b->tile[1], b->tile[2],
-                                                        w.unit_flag);
   k_unit_count<<<dim3((unsigned)nu), 256, 0, st>>>(b->tile_inst, b->n_tiles, w.unit_flag,
                                                    w.ucount);
   k_unit_rank<<<dim3((unsigned)nu), 256, 0, st>>>(b->tile_inst, b->n_tiles, w.unit_flag, w.ucount,
                                                   v.unit_rank, v.active_unit, v.n_active);
-  k_sort_key<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn,
-                                                       b->tile[0], b->tile[1], b->tile[2],
-                                                       v.unit_rank, w.key, w.slot, w.count);
+  k_sort_key<<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, TB,
+                                                   b->n_tiles, v.unit_rank, w.key, w.slot,
+                                                   w.count);
   k_scan_blocks<<<dim3((unsigned)nb), kScanBlock, 0, st>>>(w.count, v.start, w.bsum, v.n_active,
                                                             TB);
   k_scan_add<<<dim3((unsigned)nb), kScanBlock, 0, st>>>(v.start, w.bsum, v.n_active, TB);
@@ -1284,13 +1312,27 @@ int sort_impl(const mpm_batch* b, const T* dyn, int32_t* sortbuf, const Workspac
   return check_launch("sort_by_cell");
 }
 
+template <typename T>
+int sort_impl(const mpm_batch* b, const T* dyn, int32_t* sortbuf, const Workspace& w,
+              hipStream_t st) {
+  const int bs = b->block;
+  {
+  TimedLaunch tl(MPM_K_SORT, st);
+  k_cell_keys<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn,
+                                                        b->tile[0], b->tile[1], b->tile[2], w.key,
+                                                        w.unit_flag);
+  }
+  return sort_from_keys(b, sortbuf, w, st);
+}
+
 template <typename T, int TX, int TY, int TZ>
 void launch_tile_fwd(const mpm_batch* b, const SortView<const int32_t>& v, const Workspace& w,
                      const T* payload, T* grid, hipStream_t st) {
   {
   TimedLaunch tl(MPM_K_P2G_FWD, st);
-  k_p2g_tile<T, TX, TY, TZ, true><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
-      b->inst, b->tile_inst, v.active_unit, v.n_active, v.start, payload, (T*)w.partial);
+  k_p2g_tile<T, TX, TY, TZ, true, true><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
+      b->inst, b->tile_inst, v.active_unit, v.n_active, v.start, v.sorted, payload,
+      (T*)w.partial);
   }
   {
   TimedLaunch tl(MPM_K_GRID_FWD, st);
@@ -1304,8 +1346,8 @@ void launch_tile_adj(const mpm_batch* b, const SortView<const int32_t>& v, const
                      const T* grid, int advect, hipStream_t st) {
   {
   TimedLaunch tl(MPM_K_P2G_ADJ_TILE, st);
-  k_p2g_tile<T, TX, TY, TZ, false><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
-      b->inst, b->tile_inst, v.active_unit, v.n_active, v.start, (const T*)w.payload,
+  k_p2g_tile<T, TX, TY, TZ, false, false><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
+      b->inst, b->tile_inst, v.active_unit, v.n_active, v.start, v.sorted, (const T*)w.payload,
       (T*)w.partial);
   }
   {
@@ -1338,33 +1380,50 @@ void dispatch_adj(int ti, const mpm_batch* b, const SortView<const int32_t>& sv,
   }
 }
 
+// One forward step.  ready: this step's payload, sort keys and unit flags
+// were written by the previous chained step (G2P with NEXT) into `payload`
+// and the workspace; else k_prep_fwd computes them from dyn_in.
+// payload_next (optional): G2P also writes the next step's payload there,
+// and its keys and flags into the workspace, with growth factor sched_next.
 template <typename T>
-int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sched, T* dyn_out,
-                  T* grid, int32_t* sortbuf, T* payload, int advect, void* ws, void* stream) {
+int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sched,
+                  const T* sched_next, T* dyn_out, T* grid, int32_t* sortbuf, T* payload,
+                  T* payload_next, int advect, int ready, void* ws, void* stream) {
   int rc = validate(b);
   if (rc) return rc;
   if (!dyn_in || !Fg || !dyn_out || !grid || !sortbuf || !payload || !ws)
     return set_err(MPM_EINVAL, "step_fwd: NULL buffer");
   if (dyn_in == dyn_out) return set_err(MPM_EINVAL, "step_fwd: dyn_out must not alias dyn_in");
+  if (payload_next == payload) return set_err(MPM_EINVAL, "step_fwd: payload_next aliases payload");
   hipStream_t st = (hipStream_t)stream;
   Workspace w;
   carve(b, sizeof(T), (char*)ws, &w);
-  rc = sort_impl<T>(b, dyn_in, sortbuf, w, st);
+  const int bs = b->block;
+  const int TX = b->tile[0], TY = b->tile[1], TZ = b->tile[2];
+  if (!ready) {
+    TimedLaunch tl(MPM_K_PREP_FWD, st);
+    with_model(b->model_mask, [&](auto M) {
+      k_prep_fwd<T, decltype(M)::value><<<grid1(b->n_slots, bs), bs, 0, st>>>(
+          b->inst, b->slot_inst, b->n_slots, dyn_in, Fg, sched, TX, TY, TZ, payload, w.key,
+          w.unit_flag);
+    });
+  }
+  rc = sort_from_keys(b, sortbuf, w, st);
   if (rc) return rc;
   const SortView<const int32_t> sv(b, sortbuf);
-  const int32_t* rank = sv.rank;
-  const int bs = b->block;
-  {
-  TimedLaunch tl(MPM_K_PREP_FWD, st);
-  with_model(b->model_mask, [&](auto M) {
-    k_prep_fwd<T, decltype(M)::value><<<grid1(b->n_slots, bs), bs, 0, st>>>(
-        b->inst, b->slot_inst, b->n_slots, dyn_in, Fg, sched, rank, payload);
-  });
-  }
   dispatch_fwd<T>(tile_index(b), b, sv, w, payload, grid, st);
   TimedLaunch tl(MPM_K_G2P, st);
-  k_g2p<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in, grid,
-                                                 dyn_out, advect);
+  with_model(b->model_mask, [&](auto M) {
+    constexpr int MODEL = decltype(M)::value;
+    if (payload_next)
+      k_g2p<T, MODEL, true><<<grid1(b->n_slots, bs), bs, 0, st>>>(
+          b->inst, b->slot_inst, b->n_slots, dyn_in, grid, dyn_out, advect, Fg, sched_next, TX,
+          TY, TZ, payload_next, w.key, w.unit_flag);
+    else
+      k_g2p<T, MODEL, false><<<grid1(b->n_slots, bs), bs, 0, st>>>(
+          b->inst, b->slot_inst, b->n_slots, dyn_in, grid, dyn_out, advect, nullptr, nullptr, TX,
+          TY, TZ, nullptr, nullptr, nullptr);
+  });
   return check_launch("step_fwd");
 }
 
@@ -1524,14 +1583,29 @@ int32_t mpm_sort_by_cell_f64(const mpm_batch* b, const double* dyn, int32_t* sor
 int32_t mpm_step_fwd_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
                          const float* sched, float* dyn_out, float* grid, int32_t* sortbuf,
                          float* payload, int32_t advect, void* ws, void* stream) {
-  return step_fwd_impl<float>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, payload, advect, ws,
-                              stream);
+  return step_fwd_impl<float>(b, dyn_in, Fg, sched, nullptr, dyn_out, grid, sortbuf, payload,
+                              nullptr, advect, 0, ws, stream);
 }
 int32_t mpm_step_fwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
                          const double* sched, double* dyn_out, double* grid, int32_t* sortbuf,
                          double* payload, int32_t advect, void* ws, void* stream) {
-  return step_fwd_impl<double>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, payload, advect, ws,
-                               stream);
+  return step_fwd_impl<double>(b, dyn_in, Fg, sched, nullptr, dyn_out, grid, sortbuf, payload,
+                               nullptr, advect, 0, ws, stream);
+}
+int32_t mpm_step_fwd_chain_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
+                               const float* sched, const float* sched_next, float* dyn_out,
+                               float* grid, int32_t* sortbuf, float* payload, float* payload_next,
+                               int32_t advect, int32_t ready, void* ws, void* stream) {
+  return step_fwd_impl<float>(b, dyn_in, Fg, sched, sched_next, dyn_out, grid, sortbuf, payload,
+                              payload_next, advect, ready, ws, stream);
+}
+int32_t mpm_step_fwd_chain_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
+                               const double* sched, const double* sched_next, double* dyn_out,
+                               double* grid, int32_t* sortbuf, double* payload,
+                               double* payload_next, int32_t advect, int32_t ready, void* ws,
+                               void* stream) {
+  return step_fwd_impl<double>(b, dyn_in, Fg, sched, sched_next, dyn_out, grid, sortbuf, payload,
+                               payload_next, advect, ready, ws, stream);
 }
 int32_t mpm_step_bwd_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
                          const float* sched, const float* dyn_out, const float* grid,

@@ -280,6 +280,17 @@ class BatchPlan:
                stream_ptr(stream))
         check(rc, "step_fwd")
 
+    def step_fwd_chain(self, dyn_in, G, sched_row, sched_next, dyn_out, grid, sortbuf, payload,
+                       payload_next, advect, ready, stream=None):
+        """One step of a chain: ``ready`` -- this step's payload and sort keys
+        come from the previous chained step; ``payload_next`` (or None)
+        receives the next step's payload."""
+        f = getattr(_lib.lib(), f"mpm_step_fwd_chain_{suffix(dyn_in.dtype)}")
+        rc = f(ctypes.byref(self.c), ptr(dyn_in), ptr(G), sched_row, sched_next, ptr(dyn_out),
+               ptr(grid), ptr(sortbuf), ptr(payload), ptr(payload_next), int(bool(advect)),
+               int(bool(ready)), ptr(self.workspace(dyn_in.dtype)), stream_ptr(stream))
+        check(rc, "step_fwd_chain")
+
     def step_bwd(self, dyn_in, G, sched_row, dyn_out, grid, sortbuf, payload, bar_out, bar_in,
                  G_bar, theta_acc, advect, stream=None):
         f = getattr(_lib.lib(), f"mpm_step_bwd_{suffix(dyn_in.dtype)}")
@@ -365,11 +376,17 @@ class Trajectory:
         return self.ckpt[s] if k == 0 else self.seg_dyn[k - 1]
 
     def _run_segment(self, s, keep):
+        """The segment's steps as one chain: each step's G2P writes the next
+        step's payload and sort keys; the first step computes its own."""
         t0, L = self._seg(s)
         for k in range(L):
             dst = self.seg_dyn[k] if (keep or k < L - 1) else self.ckpt[s + 1]
-            self.plan.step_fwd(self._src(s, k), self.G, self._a(t0 + k), dst, self.seg_grid[k],
-                               self.seg_sort[k], self.seg_payload[k], self.advect, self.stream)
+            last = k == L - 1
+            self.plan.step_fwd_chain(self._src(s, k), self.G, self._a(t0 + k),
+                                     None if last else self._a(t0 + k + 1), dst,
+                                     self.seg_grid[k], self.seg_sort[k], self.seg_payload[k],
+                                     None if last else self.seg_payload[k + 1], self.advect,
+                                     k > 0, self.stream)
         self._loaded = s if keep else -1
 
     def forward(self, dyn0, G=None):

@@ -200,3 +200,41 @@ def test_determinism_bitwise():
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
     for a, b in zip(runs[0][2], runs[1][2]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["nh", "hencky", "hb"])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_chained_steps_bitwise_equal_single_steps(name, dtype):
+    """A trajectory runs its steps as chains (each G2P writes the next step's
+    payload and sort keys); the same steps through transfer.step one call at a
+    time (each self-contained) give bit-identical states and step records."""
+    from morphompm_amd import integrate as G, transfer as T
+    from morphompm_amd.scenes import free_swell_cube
+    from morphompm_amd.state import ParticleState
+    sc = free_swell_cube(cells=6, ppc=8, seed=5, model="hencky" if name == "hencky" else "nh")
+    m = _gpu_models()[name]
+    st0 = ParticleState.rest(torch.tensor(sc.pts, dtype=dtype, device="cuda"))
+    st0.Fg = (1.06 * torch.eye(3, dtype=dtype, device="cuda")).expand(sc.n, 3, 3).clone()
+    fin, h = G.rollout(st0, m, sc.cfg, 9, advect=True, checkpoint_every=4)
+    st = st0
+    for _ in range(9):
+        st, cache = T.step(st, m, sc.cfg, advect=True)
+    for f in ("x", "v", "C", "F"):
+        assert torch.equal(getattr(fin, f), getattr(st, f)), f
+    # the last step's records: grid, sort and payload of the kept segment
+    traj = h.traj
+    k = traj._seg(traj.nseg - 1)[1] - 1
+    assert torch.equal(traj.seg_grid[k], cache.grid)
+    pl = h.plan
+    ns, nb, U = pl.n_slots, pl.n_bins, pl.n_tiles
+    TB = int(np.prod(pl.tile))
+    for a, b in ((traj.seg_sort[k], cache.sort),):
+        na = int(b[-1])
+        assert int(a[-1]) == na
+        assert torch.equal(a[:2 * ns], b[:2 * ns])                               # rank, sorted
+        assert torch.equal(a[2 * ns:2 * ns + na * TB + 1], b[2 * ns:2 * ns + na * TB + 1])
+        u0 = 2 * ns + nb + 1
+        assert torch.equal(a[u0:u0 + U], b[u0:u0 + U])                           # unit_rank
+        assert torch.equal(a[u0 + U:u0 + U + na], b[u0 + U:u0 + U + na])         # active_unit
+    live = h.plan.live
+    assert torch.equal(traj.seg_payload[k][live], cache.payload[live])
