@@ -172,7 +172,7 @@ def main():
     tune = None
     if bi.tune is not None:
         tune = {"shape_class": bi.tune.key, "cached": bi.tune.cached,
-                "ms_per_fwd_rev_step": bi.tune.ms,
+                "ms_per_candidate_trajectory": bi.tune.ms,
                 "setup_s": round(time.perf_counter() - t_tune, 2)}
         print(f"[bench] autotuner: tile {bi.plan.tile} block {bi.plan.block} "
               f"({'cached' if bi.tune.cached else 'tuned'})", file=sys.stderr, flush=True)

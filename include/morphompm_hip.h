@@ -185,6 +185,29 @@ int32_t mpm_step_bwd_f64(const mpm_batch* b, const double* dyn_in, const double*
                          double* bar_in, double* Fg_bar, double* theta_acc, int32_t advect,
                          void* ws, void* stream);
 
+/* The same reverse step for a chain of reverse steps on one workspace and
+ * stream (steps t, t-1, ... of one checkpoint segment).  ready = 1: this
+ * step's G2P-adjoint payload is in ws from the previous chained call (that of
+ * step t+1), which must be the last call on this workspace; ready = 0: it is
+ * computed from dyn_in and bar_out first.  dyn_prev / sortbuf_prev (both or
+ * NULL): step t-1's input state and sort record; the constitutive-VJP kernel
+ * then also writes step t-1's payload into ws.  Results are bit for bit
+ * those of mpm_step_bwd.  No reference counterpart (a fusion of consecutive
+ * transfer.step_vjp calls, python/morphompm/integrate.py:24-28). */
+int32_t mpm_step_bwd_chain_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
+                               const float* sched, const float* dyn_out, const float* grid,
+                               const int32_t* sortbuf, const float* payload, const float* bar_out,
+                               float* bar_in, float* Fg_bar, double* theta_acc, int32_t advect,
+                               int32_t ready, const float* dyn_prev, const int32_t* sortbuf_prev,
+                               void* ws, void* stream);
+int32_t mpm_step_bwd_chain_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
+                               const double* sched, const double* dyn_out, const double* grid,
+                               const int32_t* sortbuf, const double* payload,
+                               const double* bar_out, double* bar_in, double* Fg_bar,
+                               double* theta_acc, int32_t advect, int32_t ready,
+                               const double* dyn_prev, const int32_t* sortbuf_prev, void* ws,
+                               void* stream);
+
 /* Per-launch timing of one kernel (profiling aid, no reference counterpart).
  * `events` holds 2*n_pairs hipEvent_t handles created with timing enabled.
  * Every later launch of kernel `kernel_id` records events[2i] just before and

@@ -77,6 +77,9 @@ def lib():
         f = getattr(L, f"mpm_step_bwd_{suf}")
         f.restype = _I32
         f.argtypes = [ctypes.POINTER(MpmBatch)] + [_P] * 11 + [_I32, _P, _P]
+        f = getattr(L, f"mpm_step_bwd_chain_{suf}")
+        f.restype = _I32
+        f.argtypes = [ctypes.POINTER(MpmBatch)] + [_P] * 11 + [_I32, _I32, _P, _P, _P, _P]
     L.mpm_kernel_timer.restype = _I32
     L.mpm_kernel_timer.argtypes = [_I32, _P, _I32]
     L.mpm_kernel_timer_count.restype = _I32

@@ -3,9 +3,10 @@
 Two launch parameters are free: the P2G tile shape (cells per P2G block; the
 sort bins particles by these tiles) and the block size of the particle
 kernels (particles per block).  ``autotune`` builds a ``BatchPlan`` for every
-candidate, runs a few forward steps of the batch's own particles from rest
-under their growth tensors, then times forward + reverse steps with HIP events
-and keeps the fastest.  Results are cached per *shape class* -- dtype, model
+candidate and times, with HIP events, a short checkpointed trajectory of the
+batch's own particles from rest under their growth tensors (chained forward
+steps, a segment recompute, chained reverse steps), keeping the fastest
+median.  Results are cached per *shape class* -- dtype, model
 mix, advection, and log2 buckets of the particle count, the instance count and
 the mean grid size -- in a JSON file, so a workload is tuned once:
 
@@ -24,7 +25,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .engine import BatchPlan
+from .engine import BatchPlan, Trajectory
 
 CANDIDATE_BLOCKS = (64, 128, 256)
 
@@ -75,7 +76,7 @@ def store_cache(path, key, entry):
 class TuneResult:
     tile: tuple
     block: int
-    ms: dict = field(default_factory=dict)   # "tx,ty,tz/block" -> ms per fwd+rev step
+    ms: dict = field(default_factory=dict)   # "tx,ty,tz/block" -> ms per short trajectory
     key: str = ""
     cached: bool = False
 
@@ -88,41 +89,36 @@ def _label(tile, block):
     return f"{tile[0]},{tile[1]},{tile[2]}/{block}"
 
 
-def _time_candidate(specs, pts_list, G_list, dtype, advect, tile, block, warm, reps):
+def _time_candidate(specs, pts_list, G_list, dtype, advect, tile, block, steps, reps):
+    """Median ms of a checkpointed trajectory of `steps` steps (chained
+    forward, segment recompute, chained reverse) -- the kernels and launch mix
+    a batched trajectory runs."""
     plan = BatchPlan(specs, tile=tile, block=block)
     perm = plan.cell_order(pts_list)
     parts = []
     for s, p in zip(specs, pts_list):
         k = s.n
         parts.append((p, np.zeros((k, 3)), np.zeros((k, 3, 3)), np.repeat(np.eye(3)[None], k, 0)))
-    a = plan.pack_dyn(parts, dtype, perm)
+    dyn0 = plan.pack_dyn(parts, dtype, perm)
     G = plan.pack_mat(G_list, dtype, perm)
-    b = plan.new_dyn(dtype)
-    grid, sb, pl = plan.new_grid(dtype), plan.new_sort(), plan.new_payload(dtype)
-    for _ in range(warm):                      # evolve the state off rest
-        plan.step_fwd(a, G, None, b, grid, sb, pl, advect)
-        a, b = b, a
-    bar_out = plan.new_dyn(dtype)
-    bar_out[15:24] = plan.live.to(dtype)       # a unit F cotangent
-    bar_in, G_bar = plan.new_dyn(dtype), plan.new_mat(dtype)
-    theta = torch.zeros(plan.n_slots // 256, 2, dtype=torch.float64, device=plan.device)
+    traj = Trajectory(plan, dtype, steps, advect, max(1, steps // 2))
+    bar = plan.new_dyn(dtype)
+    bar[15:24] = plan.live.to(dtype)           # a unit F cotangent
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-
-    def one():
-        plan.step_fwd(a, G, None, b, grid, sb, pl, advect)
-        plan.step_bwd(a, G, None, b, grid, sb, pl, bar_out, bar_in, G_bar, theta, advect)
-    one()
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(reps):
-        one()
-    e1.record()
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps
+    ms = []
+    for r in range(reps + 1):
+        e0.record()
+        traj.forward(dyn0, G)
+        traj.backward(bar)
+        e1.record()
+        torch.cuda.synchronize()
+        if r:                                  # the first run warms up
+            ms.append(e0.elapsed_time(e1))
+    return float(np.median(ms))
 
 
 def autotune(specs, pts_list, dtype=torch.float32, advect=True, G_list=None, tiles=None,
-             blocks=CANDIDATE_BLOCKS, warm=3, reps=3, cache=True, path=None):
+             blocks=CANDIDATE_BLOCKS, steps=6, reps=5, cache=True, path=None):
     """Best (tile, block) for this batch; see the module docstring."""
     _lib.require_gpu()
     key = shape_class(specs, dtype, advect)
@@ -140,7 +136,7 @@ def autotune(specs, pts_list, dtype=torch.float32, advect=True, G_list=None, til
     for tile in tiles:
         for block in blocks:
             ms[_label(tile, block)] = _time_candidate(specs, pts_list, G_list, dtype, advect, tile,
-                                                      block, warm, reps)
+                                                      block, steps, reps)
     best = min(ms, key=ms.get)
     tl, bl = best.split("/")
     res = TuneResult(tuple(int(v) for v in tl.split(",")), int(bl), ms, key, False)

@@ -851,6 +851,23 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
 //   P = nv_b, A = 4 inv_dx^2 C_b'.
 // ---------------------------------------------------------------------------
 template <typename T>
+__device__ __forceinline__ void make_adj_payload(const Inst<T>& I, const T x[3], const M3<T>& F,
+                                                 const T xb[3], const T vb[3], const M3<T>& Cnb,
+                                                 const M3<T>& Fnb, int advect, T* rec) {
+  T fx[3], nvb[3];
+  int base[3];
+  stencil_base(x, I, base, fx);
+  for (int a = 0; a < 3; ++a) nvb[a] = advect ? vb[a] + I.dt * xb[a] : vb[a];
+  const M3<T> FF = mul_nt(Fnb, F);
+  M3<T> Ap;
+  for (int k = 0; k < 9; ++k) Ap.a[k] = I.k4 * (Cnb.a[k] + I.dt * FF.a[k]) * I.dx;
+  T q[3];
+  for (int a = 0; a < 3; ++a)
+    q[a] = nvb[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
+  store_payload(rec, fx, q, Ap, pack_cell(base, I.nx, I.ny, I.nz));
+}
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_prep_adj(const mpm_instance* inst, const int32_t* slot_inst, int64_t n_slots,
                            const T* dyn_in, const T* bar_out, const int32_t* rank, T* payload,
                            int advect) {
@@ -858,27 +875,12 @@ __global__ __launch_bounds__(256) void k_prep_adj(const mpm_instance* inst, cons
   if (p >= n_slots) return;
   const Inst<T> I = load_inst<T>(inst, block_inst(slot_inst));
   if (p - I.p_off >= I.n) return;
-  T x[3], fx[3], nvb[3];
-  int base[3];
+  T x[3], xb[3], vb[3];
   ld3(dyn_in, n_slots, p, RX, x);
-  stencil_base(x, I, base, fx);
-  ld3(bar_out, n_slots, p, RV, nvb);
-  if (advect) {
-    T xb[3];
-    ld3(bar_out, n_slots, p, RX, xb);
-    for (int a = 0; a < 3; ++a) nvb[a] += I.dt * xb[a];
-  }
-  const M3<T> F = ld9(dyn_in, n_slots, p, RF);
-  const M3<T> Cnb = ld9(bar_out, n_slots, p, RC);
-  const M3<T> Fnb = ld9(bar_out, n_slots, p, RF);
-  const M3<T> FF = mul_nt(Fnb, F);
-  M3<T> Ap;
-  for (int k = 0; k < 9; ++k) Ap.a[k] = I.k4 * (Cnb.a[k] + I.dt * FF.a[k]) * I.dx;
-  T q[3];
-  for (int a = 0; a < 3; ++a)
-    q[a] = nvb[a] - (Ap(a, 0) * fx[0] + Ap(a, 1) * fx[1] + Ap(a, 2) * fx[2]);
-  store_payload(payload + (int64_t)rank[p] * kRecord, fx, q, Ap,
-                pack_cell(base, I.nx, I.ny, I.nz));
+  ld3(bar_out, n_slots, p, RX, xb);
+  ld3(bar_out, n_slots, p, RV, vb);
+  make_adj_payload(I, x, ld9(dyn_in, n_slots, p, RF), xb, vb, ld9(bar_out, n_slots, p, RC),
+                   ld9(bar_out, n_slots, p, RF), advect, payload + (int64_t)rank[p] * kRecord);
 }
 
 // reverse step, part 2 (transfer.py:121-127): grid adjoint.
@@ -1054,12 +1056,17 @@ __global__ __launch_bounds__(256) void k_adj_gather(const mpm_instance* inst,
 // per particle slot.  tau_b = c sk_b with sk_b = dt MD, c = -4 inv_dx^2 vol0 det Fg;
 //   C_bar = pmass MD + C_bar(stress),  F_bar = (I + dt Cn)^T Fn_b + F_bar(stress),
 //   G_bar += a (Fg_bar(stress) + <sk_b, tau> (-4 inv_dx^2 vol0) det Fg Fg^-T).
-template <typename T, int MODEL>
+// PREV: with the state's full cotangent at hand (x_bar, v_bar from the gather
+// kernel), also writes the previous step's G2P-adjoint payload, what
+// k_prep_adj computes for that step, at its sorted position rank_prev[p].
+template <typename T, int MODEL, bool PREV>
 __global__ __launch_bounds__(256) void k_adj_vjp(const mpm_instance* inst, const int32_t* slot_inst,
                                                  int64_t n_slots, const int32_t* rank,
                                                  const T* dyn_in, const T* Fgp, const T* sched,
                                                  const T* dyn_out, const T* md, const T* bar_out,
-                                                 T* bar_in, T* Fg_bar, double* theta_acc) {
+                                                 T* bar_in, T* Fg_bar, double* theta_acc,
+                                                 const T* dyn_prev, const int32_t* rank_prev,
+                                                 T* payload_prev, int advect) {
   __shared__ double red[2][256];
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int ii = block_inst(slot_inst);
@@ -1079,11 +1086,9 @@ __global__ __launch_bounds__(256) void k_adj_vjp(const mpm_instance* inst, const
       const M3<T> F = ld9(dyn_in, n_slots, p, RF);
       kirchhoff_vjp(I.mp, F, Fg, C, scaled(skb, c), Fpb, Fgb, Cbar, th, &tau);
     }
-    {
-      M3<T> Cbo = scaled(MD, I.pmass);
-      for (int k = 0; k < 9; ++k) Cbo.a[k] += Cbar.a[k];
-      st9(bar_in, n_slots, p, RC, Cbo);
-    }
+    M3<T> Cbo = scaled(MD, I.pmass);
+    for (int k = 0; k < 9; ++k) Cbo.a[k] += Cbar.a[k];
+    st9(bar_in, n_slots, p, RC, Cbo);
     {
       const M3<T> Cn = ld9(dyn_out, n_slots, p, RC);
       const M3<T> Fnb = ld9(bar_out, n_slots, p, RF);
@@ -1091,6 +1096,14 @@ __global__ __launch_bounds__(256) void k_adj_vjp(const mpm_instance* inst, const
       const M3<T> CtF = mul_tn(Cn, Fnb);
       for (int k = 0; k < 9; ++k) Fpb.a[k] += Fnb.a[k] + I.dt * CtF.a[k];
       st9(bar_in, n_slots, p, RF, Fpb);
+    }
+    if (PREV) {
+      T xp[3], xb[3], vb[3];
+      ld3(dyn_prev, n_slots, p, RX, xp);
+      ld3(bar_in, n_slots, p, RX, xb);
+      ld3(bar_in, n_slots, p, RV, vb);
+      make_adj_payload(I, xp, ld9(dyn_prev, n_slots, p, RF), xb, vb, Cbo, Fpb, advect,
+                       payload_prev + (int64_t)rank_prev[p] * kRecord);
     }
     const M3<T> GiT = inv_t(Fg);
     const T vol = frob(skb, tau) * (-I.k4 * I.vol0) * detG;
@@ -1427,27 +1440,34 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   return check_launch("step_fwd");
 }
 
+// One reverse step.  ready: this step's G2P-adjoint payload was written into
+// the workspace by the previous chained reverse call (the later step's);
+// else k_prep_adj computes it.  dyn_prev / sortbuf_prev (both or neither):
+// the previous step's input state and sort record; the VJP kernel then also
+// writes that step's G2P-adjoint payload for the next chained call.
 template <typename T>
 int step_bwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sched,
                   const T* dyn_out, const T* grid, const int32_t* sortbuf, const T* payload,
-                  const T* bar_out, T* bar_in, T* Fg_bar, double* theta_acc, int advect, void* ws,
-                  void* stream) {
+                  const T* bar_out, T* bar_in, T* Fg_bar, double* theta_acc, int advect, int ready,
+                  const T* dyn_prev, const int32_t* sortbuf_prev, void* ws, void* stream) {
   int rc = validate(b);
   if (rc) return rc;
   if (!dyn_in || !Fg || !dyn_out || !grid || !sortbuf || !payload || !bar_out || !bar_in ||
       !Fg_bar || !ws)
     return set_err(MPM_EINVAL, "step_bwd: NULL buffer");
   if (bar_in == bar_out) return set_err(MPM_EINVAL, "step_bwd: bar_in must not alias bar_out");
+  if ((dyn_prev == nullptr) != (sortbuf_prev == nullptr))
+    return set_err(MPM_EINVAL, "step_bwd: dyn_prev and sortbuf_prev go together");
   hipStream_t st = (hipStream_t)stream;
   Workspace w;
   carve(b, sizeof(T), (char*)ws, &w);
   const SortView<const int32_t> sv(b, sortbuf);
   const int32_t* rank = sv.rank;
   const int bs = b->block;
-  {
-  TimedLaunch tl(MPM_K_PREP_ADJ, st);
-  k_prep_adj<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in,
-                                                      bar_out, rank, (T*)w.payload, advect);
+  if (!ready) {
+    TimedLaunch tl(MPM_K_PREP_ADJ, st);
+    k_prep_adj<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in,
+                                                        bar_out, rank, (T*)w.payload, advect);
   }
   dispatch_adj<T>(tile_index(b), b, sv, w, grid, advect, st);
   {
@@ -1458,9 +1478,16 @@ int step_bwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   }
   TimedLaunch tl(MPM_K_ADJ_VJP, st);
   with_model(b->model_mask, [&](auto M) {
-    k_adj_vjp<T, decltype(M)::value><<<grid1(b->n_slots, 256), 256, 0, st>>>(
-        b->inst, b->slot_inst, b->n_slots, rank, dyn_in, Fg, sched, dyn_out, (const T*)w.md,
-        bar_out, bar_in, Fg_bar, theta_acc);
+    constexpr int MODEL = decltype(M)::value;
+    if (dyn_prev)
+      k_adj_vjp<T, MODEL, true><<<grid1(b->n_slots, 256), 256, 0, st>>>(
+          b->inst, b->slot_inst, b->n_slots, rank, dyn_in, Fg, sched, dyn_out, (const T*)w.md,
+          bar_out, bar_in, Fg_bar, theta_acc, dyn_prev, SortView<const int32_t>(b, sortbuf_prev).rank,
+          (T*)w.payload, advect);
+    else
+      k_adj_vjp<T, MODEL, false><<<grid1(b->n_slots, 256), 256, 0, st>>>(
+          b->inst, b->slot_inst, b->n_slots, rank, dyn_in, Fg, sched, dyn_out, (const T*)w.md,
+          bar_out, bar_in, Fg_bar, theta_acc, nullptr, nullptr, nullptr, advect);
   });
   return check_launch("step_bwd");
 }
@@ -1613,7 +1640,7 @@ int32_t mpm_step_bwd_f32(const mpm_batch* b, const float* dyn_in, const float* F
                          float* bar_in, float* Fg_bar, double* theta_acc, int32_t advect, void* ws,
                          void* stream) {
   return step_bwd_impl<float>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, payload, bar_out,
-                              bar_in, Fg_bar, theta_acc, advect, ws, stream);
+                              bar_in, Fg_bar, theta_acc, advect, 0, nullptr, nullptr, ws, stream);
 }
 int32_t mpm_step_bwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
                          const double* sched, const double* dyn_out, const double* grid,
@@ -1621,7 +1648,28 @@ int32_t mpm_step_bwd_f64(const mpm_batch* b, const double* dyn_in, const double*
                          double* bar_in, double* Fg_bar, double* theta_acc, int32_t advect,
                          void* ws, void* stream) {
   return step_bwd_impl<double>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, payload, bar_out,
-                               bar_in, Fg_bar, theta_acc, advect, ws, stream);
+                               bar_in, Fg_bar, theta_acc, advect, 0, nullptr, nullptr, ws, stream);
+}
+int32_t mpm_step_bwd_chain_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
+                               const float* sched, const float* dyn_out, const float* grid,
+                               const int32_t* sortbuf, const float* payload, const float* bar_out,
+                               float* bar_in, float* Fg_bar, double* theta_acc, int32_t advect,
+                               int32_t ready, const float* dyn_prev, const int32_t* sortbuf_prev,
+                               void* ws, void* stream) {
+  return step_bwd_impl<float>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, payload, bar_out,
+                              bar_in, Fg_bar, theta_acc, advect, ready, dyn_prev, sortbuf_prev, ws,
+                              stream);
+}
+int32_t mpm_step_bwd_chain_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
+                               const double* sched, const double* dyn_out, const double* grid,
+                               const int32_t* sortbuf, const double* payload,
+                               const double* bar_out, double* bar_in, double* Fg_bar,
+                               double* theta_acc, int32_t advect, int32_t ready,
+                               const double* dyn_prev, const int32_t* sortbuf_prev, void* ws,
+                               void* stream) {
+  return step_bwd_impl<double>(b, dyn_in, Fg, sched, dyn_out, grid, sortbuf, payload, bar_out,
+                               bar_in, Fg_bar, theta_acc, advect, ready, dyn_prev, sortbuf_prev, ws,
+                               stream);
 }
 
 }  // extern "C"

@@ -299,6 +299,19 @@ class BatchPlan:
                int(bool(advect)), ptr(self.workspace(dyn_in.dtype)), stream_ptr(stream))
         check(rc, "step_bwd")
 
+    def step_bwd_chain(self, dyn_in, G, sched_row, dyn_out, grid, sortbuf, payload, bar_out,
+                       bar_in, G_bar, theta_acc, advect, ready, dyn_prev, sortbuf_prev,
+                       stream=None):
+        """One reverse step of a chain: ``ready`` -- its adjoint payload comes
+        from the previous (later) chained step; ``dyn_prev``/``sortbuf_prev``
+        (or None) -- also prepare the previous step's."""
+        f = getattr(_lib.lib(), f"mpm_step_bwd_chain_{suffix(dyn_in.dtype)}")
+        rc = f(ctypes.byref(self.c), ptr(dyn_in), ptr(G), sched_row, ptr(dyn_out), ptr(grid),
+               ptr(sortbuf), ptr(payload), ptr(bar_out), ptr(bar_in), ptr(G_bar), ptr(theta_acc),
+               int(bool(advect)), int(bool(ready)), ptr(dyn_prev), ptr(sortbuf_prev),
+               ptr(self.workspace(dyn_in.dtype)), stream_ptr(stream))
+        check(rc, "step_bwd_chain")
+
     def sort_by_cell(self, dyn, sortbuf, stream=None):
         f = getattr(_lib.lib(), f"mpm_sort_by_cell_{suffix(dyn.dtype)}")
         check(f(ctypes.byref(self.c), ptr(dyn), ptr(sortbuf), ptr(self.workspace(dyn.dtype)),
@@ -430,11 +443,15 @@ class Trajectory:
             if self._loaded != s:
                 self._run_segment(s, keep=True)
             t0, L = self._seg(s)
+            # one chain per segment: each reverse step's VJP kernel prepares
+            # the adjoint payload of the step before it
             for k in range(L - 1, -1, -1):
-                self.plan.step_bwd(self._src(s, k), self.G, self._a(t0 + k), self.seg_dyn[k],
-                                   self.seg_grid[k], self.seg_sort[k], self.seg_payload[k],
-                                   bar_out, bar_in,
-                                   self.G_bar, th, self.advect, self.stream)
+                self.plan.step_bwd_chain(self._src(s, k), self.G, self._a(t0 + k),
+                                         self.seg_dyn[k], self.seg_grid[k], self.seg_sort[k],
+                                         self.seg_payload[k], bar_out, bar_in, self.G_bar, th,
+                                         self.advect, k < L - 1,
+                                         self._src(s, k - 1) if k > 0 else None,
+                                         self.seg_sort[k - 1] if k > 0 else None, self.stream)
                 bar_out, bar_in = bar_in, bar_out
         theta = self.plan.theta_per_instance(self.theta_acc) if want_theta else None
         return bar_out, self.G_bar, theta

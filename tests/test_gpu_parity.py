@@ -217,18 +217,21 @@ def test_chained_steps_bitwise_equal_single_steps(name, dtype):
     st0.Fg = (1.06 * torch.eye(3, dtype=dtype, device="cuda")).expand(sc.n, 3, 3).clone()
     fin, h = G.rollout(st0, m, sc.cfg, 9, advect=True, checkpoint_every=4)
     st = st0
+    caches = []
     for _ in range(9):
         st, cache = T.step(st, m, sc.cfg, advect=True)
+        caches.append(cache)
     for f in ("x", "v", "C", "F"):
         assert torch.equal(getattr(fin, f), getattr(st, f)), f
-    # the last step's records: grid, sort and payload of the kept segment
+    # the last step's records (grid, sort, payload of the kept segment), read
+    # before the reverse pass recomputes earlier segments into the same store
     traj = h.traj
     k = traj._seg(traj.nseg - 1)[1] - 1
-    assert torch.equal(traj.seg_grid[k], cache.grid)
+    assert torch.equal(traj.seg_grid[k], caches[-1].grid)
     pl = h.plan
     ns, nb, U = pl.n_slots, pl.n_bins, pl.n_tiles
     TB = int(np.prod(pl.tile))
-    for a, b in ((traj.seg_sort[k], cache.sort),):
+    for a, b in ((traj.seg_sort[k], caches[-1].sort),):
         na = int(b[-1])
         assert int(a[-1]) == na
         assert torch.equal(a[:2 * ns], b[:2 * ns])                               # rank, sorted
@@ -237,4 +240,16 @@ def test_chained_steps_bitwise_equal_single_steps(name, dtype):
         assert torch.equal(a[u0:u0 + U], b[u0:u0 + U])                           # unit_rank
         assert torch.equal(a[u0 + U:u0 + U + na], b[u0 + U:u0 + U + na])         # active_unit
     live = h.plan.live
-    assert torch.equal(traj.seg_payload[k][live], cache.payload[live])
+    assert torch.equal(traj.seg_payload[k][live], caches[-1].payload[live])
+    # reverse: chained (each VJP kernel prepares the step before) vs one
+    # self-contained step_vjp call per step
+    g = torch.Generator(device="cuda").manual_seed(7)
+    n = sc.n
+    bars = [torch.randn(n, *sh, dtype=dtype, device="cuda", generator=g)
+            for sh in ((3,), (3,), (3, 3), (3, 3), (3, 3))]
+    chained = G.rollout_vjp(h, *bars, advect=True)
+    single = bars
+    for c in reversed(caches):
+        single = T.step_vjp(c, *single, advect=True)
+    for a, b in zip(chained, single):
+        assert torch.equal(a, b)
