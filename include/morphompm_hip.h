@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 3
+#define MPM_ABI_VERSION 4
 #define MPM_SLOT_ALIGN 256 /* particle and node ranges of an instance start at multiples of this */
 
 enum { MPM_OK = 0, MPM_EINVAL = -1, MPM_ELAUNCH = -2, MPM_ENOTSUP = -3 };
@@ -94,6 +94,12 @@ typedef struct mpm_batch {
   const int32_t* slot_inst;  /* device [n_slots / 256]: instance owning each 256 slots */
   const int32_t* node_inst;  /* device [n_nodes / 256] */
   const int32_t* tile_inst;  /* device [n_tiles]: instance owning each unit */
+  /* Optional (NULL: one launch of mixed-model kernels): device
+   * [n_slots / 256], the 256-slot chunks grouped by model -- model m's chunks
+   * are model_chunk[model_chunk_off[m] .. model_chunk_off[m + 1]).  The
+   * model-dependent kernels then run once per model, specialised for it. */
+  const int32_t* model_chunk;
+  int64_t model_chunk_off[4];
 } mpm_batch;
 
 int32_t mpm_abi_version(void);

@@ -11,7 +11,7 @@ import torch
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libmorphompm_hip.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 SLOT_ALIGN = 256
 
 NEO_HOOKEAN, HENCKY, HERSCHEL_BULKLEY = 0, 1, 2
@@ -34,7 +34,8 @@ class MpmInstance(ctypes.Structure):
 class MpmBatch(ctypes.Structure):
     _fields_ = [("n_inst", _I32), ("tile", _I32 * 3), ("n_slots", _I64), ("n_nodes", _I64),
                 ("n_bins", _I64), ("n_tiles", _I64), ("block", _I32), ("model_mask", _I32),
-                ("inst", _P), ("slot_inst", _P), ("node_inst", _P), ("tile_inst", _P)]
+                ("inst", _P), ("slot_inst", _P), ("node_inst", _P), ("tile_inst", _P),
+                ("model_chunk", _P), ("model_chunk_off", _I64 * 4)]
 
 
 class MpmError(RuntimeError):

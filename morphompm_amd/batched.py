@@ -15,6 +15,7 @@ applied per instance.
 import numpy as np
 import torch
 
+from .diff import gauss_newton_update
 from .engine import BatchPlan, InstanceSpec, Trajectory, DEFAULT_TILE, ROW_F
 
 
@@ -163,15 +164,10 @@ class BatchedInverse:
             L, dL = (t.cpu().numpy() for t in self.loss_and_grad())
             for i in np.nonzero(~done)[0]:
                 its[i] = it + 1
-                resid = L[i] - targets[i]
-                if abs(resid) < tol:
-                    done[i] = True
-                    continue
-                if abs(dL[i]) < 1e-12 or not np.isfinite(dL[i]):
-                    theta[i] += 1e-3 * (1.0 if resid < 0 else -1.0)
-                    continue
-                theta[i] -= max(-0.5, min(0.5, resid / dL[i]))
-                hist[i].append(theta[i])
+                theta[i], done[i], appended = gauss_newton_update(theta[i], L[i], dL[i],
+                                                                  targets[i], tol)
+                if appended:
+                    hist[i].append(theta[i])
             if done.all():
                 break
         return (theta, its, hist) if return_history else (theta, its)

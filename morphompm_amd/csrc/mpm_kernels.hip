@@ -172,6 +172,15 @@ __device__ __forceinline__ int block_inst(const int32_t* chunk_inst) {
   return __builtin_amdgcn_readfirstlane(chunk_inst[first / kChunk]);
 }
 
+// First slot of this block.  chunks == NULL: blocks cover the slots in
+// order; else block b works in the 256-slot chunk chunks[b / (256 / blockDim)]
+// (a launch over one model's chunks).
+__device__ __forceinline__ int64_t block_first_slot(const int32_t* chunks) {
+  const int64_t first = (int64_t)blockIdx.x * blockDim.x;
+  if (chunks == nullptr) return first;
+  return (int64_t)__builtin_amdgcn_readfirstlane(chunks[first / kChunk]) * kChunk + first % kChunk;
+}
+
 // dyn row offsets
 constexpr int RX = 0, RV = 3, RC = 6, RF = 15;
 
@@ -467,10 +476,10 @@ __global__ __launch_bounds__(256) void k_prep_fwd(const mpm_instance* inst,
                                                   const int32_t* slot_inst, int64_t n_slots,
                                                   const T* dyn, const T* Fgp, const T* sched,
                                                   int TX, int TY, int TZ, T* payload, int32_t* key,
-                                                  int32_t* unit_flag) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                  int32_t* unit_flag, const int32_t* chunks) {
+  const int64_t p0 = block_first_slot(chunks), p = p0 + threadIdx.x;
   if (p >= n_slots) return;
-  const int ii = block_inst(slot_inst);
+  const int ii = __builtin_amdgcn_readfirstlane(slot_inst[p0 / kChunk]);
   const Inst<T> I = load_inst<T, MODEL>(inst, ii);
   if (p - I.p_off >= I.n) return;
   T x[3], v[3];
@@ -774,10 +783,11 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
                                              int64_t n_slots, const T* dyn_in, const T* grid,
                                              T* dyn_out, int advect, const T* Fgp,
                                              const T* sched_next, int TX, int TY, int TZ,
-                                             T* payload_next, int32_t* key, int32_t* unit_flag) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                             T* payload_next, int32_t* key, int32_t* unit_flag,
+                                             const int32_t* chunks) {
+  const int64_t p0 = block_first_slot(chunks), p = p0 + threadIdx.x;
   if (p >= n_slots) return;
-  const int ii = block_inst(slot_inst);
+  const int ii = __builtin_amdgcn_readfirstlane(slot_inst[p0 / kChunk]);
   const Inst<T> I = load_inst<T, MODEL>(inst, ii);
   if (p - I.p_off >= I.n) return;
   T x[3], fx[3];
@@ -1066,10 +1076,11 @@ __global__ __launch_bounds__(256) void k_adj_vjp(const mpm_instance* inst, const
                                                  const T* dyn_out, const T* md, const T* bar_out,
                                                  T* bar_in, T* Fg_bar, double* theta_acc,
                                                  const T* dyn_prev, const int32_t* rank_prev,
-                                                 T* payload_prev, int advect) {
+                                                 T* payload_prev, int advect,
+                                                 const int32_t* chunks) {
   __shared__ double red[2][256];
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int ii = block_inst(slot_inst);
+  const int64_t p0 = block_first_slot(chunks), p = p0 + threadIdx.x;  // blockDim = 256
+  const int ii = __builtin_amdgcn_readfirstlane(slot_inst[p0 / kChunk]);
   const Inst<T> I = load_inst<T, MODEL>(inst, ii);
   const bool live = (p < n_slots) && (p - I.p_off < I.n);
   T th[2] = {T(0), T(0)};
@@ -1123,8 +1134,8 @@ __global__ __launch_bounds__(256) void k_adj_vjp(const mpm_instance* inst, const
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      theta_acc[2 * blockIdx.x] += red[0][0];
-      theta_acc[2 * blockIdx.x + 1] += red[1][0];
+      theta_acc[2 * (p0 / kChunk)] += red[0][0];
+      theta_acc[2 * (p0 / kChunk) + 1] += red[1][0];
     }
   }
 }
@@ -1249,6 +1260,11 @@ int validate(const mpm_batch* b) {
   if (b->n_inst > 0 && b->inst == nullptr) return set_err(MPM_EINVAL, "batch tables missing");
   if (tile_index(b) < 0)
     return set_err(MPM_ENOTSUP, "tile shape %dx%dx%d not built", b->tile[0], b->tile[1], b->tile[2]);
+  if (b->model_chunk != nullptr) {
+    const int64_t* o = b->model_chunk_off;
+    if (o[0] != 0 || o[1] < o[0] || o[2] < o[1] || o[3] < o[2] || o[3] != b->n_slots / kChunk)
+      return set_err(MPM_EINVAL, "model_chunk_off must rise from 0 to n_slots / %d", kChunk);
+  }
   return MPM_OK;
 }
 
@@ -1263,6 +1279,27 @@ void with_model(int32_t mask, F&& f) {
     case 1 << MPM_HENCKY: f(std::integral_constant<int, MPM_HENCKY>()); break;
     case 1 << MPM_HERSCHEL_BULKLEY: f(std::integral_constant<int, MPM_HERSCHEL_BULKLEY>()); break;
     default: f(std::integral_constant<int, -1>()); break;
+  }
+}
+
+// Launches f(std::integral_constant<int, M>, chunks, grid) for the batch's
+// particle slots: one launch per model over that model's chunks when the
+// batch lists them (each launch runs kernels specialised for its model), else
+// one launch over all slots (chunks = NULL).
+template <typename F>
+void per_model(const mpm_batch* b, int bs, F&& f) {
+  if (b->model_chunk == nullptr || __builtin_popcount(b->model_mask) < 2) {
+    with_model(b->model_mask, [&](auto M) { f(M, (const int32_t*)nullptr, grid1(b->n_slots, bs)); });
+    return;
+  }
+  for (int m = 0; m < 3; ++m) {
+    const int64_t n = b->model_chunk_off[m + 1] - b->model_chunk_off[m];
+    if (n <= 0) continue;
+    const int32_t* ch = b->model_chunk + b->model_chunk_off[m];
+    const dim3 g((unsigned)(n * (kChunk / bs)));
+    if (m == MPM_NEO_HOOKEAN) f(std::integral_constant<int, MPM_NEO_HOOKEAN>(), ch, g);
+    if (m == MPM_HENCKY) f(std::integral_constant<int, MPM_HENCKY>(), ch, g);
+    if (m == MPM_HERSCHEL_BULKLEY) f(std::integral_constant<int, MPM_HERSCHEL_BULKLEY>(), ch, g);
   }
 }
 
@@ -1415,10 +1452,10 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   const int TX = b->tile[0], TY = b->tile[1], TZ = b->tile[2];
   if (!ready) {
     TimedLaunch tl(MPM_K_PREP_FWD, st);
-    with_model(b->model_mask, [&](auto M) {
-      k_prep_fwd<T, decltype(M)::value><<<grid1(b->n_slots, bs), bs, 0, st>>>(
+    per_model(b, bs, [&](auto M, const int32_t* ch, dim3 g) {
+      k_prep_fwd<T, decltype(M)::value><<<g, bs, 0, st>>>(
           b->inst, b->slot_inst, b->n_slots, dyn_in, Fg, sched, TX, TY, TZ, payload, w.key,
-          w.unit_flag);
+          w.unit_flag, ch);
     });
   }
   rc = sort_from_keys(b, sortbuf, w, st);
@@ -1426,17 +1463,16 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   const SortView<const int32_t> sv(b, sortbuf);
   dispatch_fwd<T>(tile_index(b), b, sv, w, payload, grid, st);
   TimedLaunch tl(MPM_K_G2P, st);
-  with_model(b->model_mask, [&](auto M) {
-    constexpr int MODEL = decltype(M)::value;
-    if (payload_next)
-      k_g2p<T, MODEL, true><<<grid1(b->n_slots, bs), bs, 0, st>>>(
+  if (payload_next)
+    per_model(b, bs, [&](auto M, const int32_t* ch, dim3 g) {
+      k_g2p<T, decltype(M)::value, true><<<g, bs, 0, st>>>(
           b->inst, b->slot_inst, b->n_slots, dyn_in, grid, dyn_out, advect, Fg, sched_next, TX,
-          TY, TZ, payload_next, w.key, w.unit_flag);
-    else
-      k_g2p<T, MODEL, false><<<grid1(b->n_slots, bs), bs, 0, st>>>(
-          b->inst, b->slot_inst, b->n_slots, dyn_in, grid, dyn_out, advect, nullptr, nullptr, TX,
-          TY, TZ, nullptr, nullptr, nullptr);
-  });
+          TY, TZ, payload_next, w.key, w.unit_flag, ch);
+    });
+  else   // the G2P part does not depend on the model
+    k_g2p<T, MPM_NEO_HOOKEAN, false><<<grid1(b->n_slots, bs), bs, 0, st>>>(
+        b->inst, b->slot_inst, b->n_slots, dyn_in, grid, dyn_out, advect, nullptr, nullptr, TX,
+        TY, TZ, nullptr, nullptr, nullptr, nullptr);
   return check_launch("step_fwd");
 }
 
@@ -1477,17 +1513,17 @@ int step_bwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
       (const T*)w.gadj, bar_out, bar_in, (T*)w.md, advect);
   }
   TimedLaunch tl(MPM_K_ADJ_VJP, st);
-  with_model(b->model_mask, [&](auto M) {
+  per_model(b, 256, [&](auto M, const int32_t* ch, dim3 g) {
     constexpr int MODEL = decltype(M)::value;
     if (dyn_prev)
-      k_adj_vjp<T, MODEL, true><<<grid1(b->n_slots, 256), 256, 0, st>>>(
+      k_adj_vjp<T, MODEL, true><<<g, 256, 0, st>>>(
           b->inst, b->slot_inst, b->n_slots, rank, dyn_in, Fg, sched, dyn_out, (const T*)w.md,
           bar_out, bar_in, Fg_bar, theta_acc, dyn_prev, SortView<const int32_t>(b, sortbuf_prev).rank,
-          (T*)w.payload, advect);
+          (T*)w.payload, advect, ch);
     else
-      k_adj_vjp<T, MODEL, false><<<grid1(b->n_slots, 256), 256, 0, st>>>(
+      k_adj_vjp<T, MODEL, false><<<g, 256, 0, st>>>(
           b->inst, b->slot_inst, b->n_slots, rank, dyn_in, Fg, sched, dyn_out, (const T*)w.md,
-          bar_out, bar_in, Fg_bar, theta_acc, nullptr, nullptr, nullptr, advect);
+          bar_out, bar_in, Fg_bar, theta_acc, nullptr, nullptr, nullptr, advect, ch);
   });
   return check_launch("step_bwd");
 }

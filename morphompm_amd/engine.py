@@ -136,6 +136,18 @@ class BatchPlan:
         c.slot_inst = self.d_slot_inst.data_ptr()
         c.node_inst = self.d_node_inst.data_ptr()
         c.tile_inst = self.d_tile_inst.data_ptr()
+        # chunks grouped by model: model-specialised kernels, one launch per model
+        kinds = [int(s.kind) for s in self.specs]
+        self.d_model_chunk = None
+        if len(set(kinds)) > 1:
+            chunk_kind = np.array([kinds[i] for i in slot_inst], dtype=np.int64)
+            order = np.argsort(chunk_kind, kind="stable").astype(np.int32)
+            counts = np.bincount(chunk_kind, minlength=3)
+            self.d_model_chunk = torch.from_numpy(order).to(self.device)
+            c.model_chunk = self.d_model_chunk.data_ptr()
+            off = np.concatenate([[0], np.cumsum(counts)])
+            for m in range(4):
+                c.model_chunk_off[m] = int(off[m])
         self.c = c
         self.sortlen = 2 * self.n_slots + self.n_bins + 1 + 2 * self.n_tiles + 1
         self._ws = {}

@@ -88,3 +88,22 @@ def tip_deflection(x, tip_mask):
 
 def sum_det(F):
     return torch.linalg.det(F).sum()
+
+
+def bilayer_curvature(x, n_bins=9):
+    """Curvature of a bent strip from particle positions (n, 3), the way the
+    reference measures it (tests/test_growth.cpp:266-295): the centreline is
+    the mean y per x-bin (n_bins bins over the x extent), fitted by least
+    squares with y = a xc^2 + b xc + c around the mean bin x; kappa = |2a|."""
+    x = np.asarray(x.detach().cpu() if isinstance(x, torch.Tensor) else x, dtype=np.float64)
+    lo, hi = x[:, 0].min(), x[:, 0].max()
+    w = (hi - lo) / n_bins + 1e-9
+    b = np.clip(((x[:, 0] - lo) / w).astype(np.int64), 0, n_bins - 1)
+    cnt = np.bincount(b, minlength=n_bins)
+    used = cnt > 0
+    xb = np.bincount(b, x[:, 0], n_bins)[used] / cnt[used]
+    yb = np.bincount(b, x[:, 1], n_bins)[used] / cnt[used]
+    xc = xb - xb.mean()
+    a = np.linalg.lstsq(np.stack([xc * xc, xc, np.ones_like(xc)], 1), yb, rcond=None)[0][0]
+    return abs(2.0 * a)
+

@@ -212,3 +212,43 @@ def trilinear_weights(pts, ctrl):
                 idx[:, k] = ((i0[:, 0] + a) * ctrl + (i0[:, 1] + b)) * ctrl + (i0[:, 2] + c)
                 k += 1
     return W, idx
+
+
+def timoshenko_bilayer(eps, axial=False, res=1, n_steps=4000):
+    """The reference's bilayer-bending scene (tests/test_growth.cpp:242-263):
+    domain 1.2 (48 res cells, dx = 0.025 / res), beam L = 0.5, h = 0.05, depth
+    0.025 centred at 0.6, particles on a lattice of spacing dx / 2 (the
+    reference accumulates the coordinates in float32, reproduced here), E = 1e4,
+    nu = 0.3, rho = 1000, damping 0.05, dt = 1e-3; the top half grows 1 + eps
+    (isotropically, or along x only with ``axial``), the bottom half not.
+    Hencky.  extra["kappa_timoshenko"] = 1.5 eps / h."""
+    from . import constitutive as K
+    sp = np.float32(0.0125 / res)
+    c = np.float32(0.6)
+
+    def axis(half):
+        v, out = np.float32(c - np.float32(half)), []
+        while v <= np.float32(c + np.float32(half) + np.float32(1e-6)):
+            out.append(float(v))
+            v = np.float32(v + sp)
+        return np.array(out)
+    xs, ys, zs = axis(0.25), axis(0.025), axis(0.0125)
+    pts = np.stack(np.meshgrid(xs, ys, zs, indexing="ij"), -1).reshape(-1, 3)
+    mat = Material(E=1.0e4, nu=0.3, density=1000.0)
+    cfg = SimConfig(N=48 * res, dx=0.025 / res, dt=1.0e-3, damping=0.05, sp=float(sp),
+                    material=mat)
+    n = len(pts)
+    top = pts[:, 1] >= float(c)
+    g = np.where(top, 1.0 + eps, 1.0)
+    G = np.zeros((n, 3, 3))
+    G[:, 0, 0] = g
+    G[:, 1, 1] = 1.0 if axial else g
+    G[:, 2, 2] = 1.0 if axial else g
+    dG = np.zeros((n, 3, 3))
+    dG[:, 0, 0] = top
+    if not axial:
+        dG[:, 1, 1] = top
+        dG[:, 2, 2] = top
+    return Scene("timoshenko_bilayer", pts, cfg, K.Hencky(mat), G, dG, n_steps, sched=None,
+                 advect=True, extra={"eps": eps, "kappa_timoshenko": 1.5 * eps / 0.05})
+

@@ -98,3 +98,20 @@ def test_gpu_fp32_parity_scene_matches_cpp():
     assert np.abs(fin.F - Fc).max() < 1e-2
     assert abs(np.linalg.det(fin.F).mean() - np.linalg.det(Fc).mean()) / abs(np.linalg.det(Fc).mean()) < 1e-3
     assert np.abs(fin.x - xc).max() < 1e-3
+
+
+@need_ref
+def test_timoshenko_scene_reproduces_fixture():
+    """The reference's bilayer-bending scene run by its own C++ solver gives the
+    curvatures of outputs/figures/bending_timoshenko.csv (copied to
+    tests/golden): this pins the scene and the curvature fit the GPU test uses."""
+    from morphompm_amd.growth import bilayer_curvature
+    from morphompm_amd.scenes import timoshenko_bilayer
+    fx = np.loadtxt(os.path.join(ROOT, "tests", "golden", "bending_timoshenko.csv"),
+                    delimiter=",", skiprows=1)
+    for eps, k_sim, k_t in fx:
+        sc = timoshenko_bilayer(eps)
+        x, _ = cpp_run(sc.pts, sc.G, 48, 0.025, 4000, 1e-3, E=1e4, nu=0.3, damping=0.05,
+                       density=1000.0, sp=0.0125)
+        assert abs(bilayer_curvature(x) - k_sim) <= 1e-3
+        assert sc.extra["kappa_timoshenko"] == pytest.approx(k_t)
