@@ -388,7 +388,10 @@ __global__ void k_place(const int32_t* key, const int32_t* slot, const int32_t* 
 // Stable order within each bin: the particle at placed position j goes to
 // start[k] + (number of its bin's particles with a smaller slot), so every
 // bin ends up ascending by slot; rank is the inverse permutation.  Work per
-// bin is quadratic in its size, with independent loads.
+// bin is quadratic in its size, with independent loads, so a bin of more
+// than kStableBin particles (no physical cell holds that many; a state that
+// has blown up can put everything in one bin) keeps its placement order.
+constexpr int32_t kStableBin = 2048;
 __global__ __launch_bounds__(256) void k_bin_rank(const int32_t* key, const int32_t* start,
                                                   const int32_t* placed, int32_t* sorted,
                                                   int32_t* rank, int64_t n_slots) {
@@ -399,9 +402,11 @@ __global__ __launch_bounds__(256) void k_bin_rank(const int32_t* key, const int3
   int32_t r = (int32_t)j;
   if (k >= 0) {
     const int32_t s0 = start[k], s1 = start[k + 1];
-    r = s0;
+    if (s1 - s0 <= kStableBin) {
+      r = s0;
 #pragma unroll 4
-    for (int32_t i = s0; i < s1; ++i) r += placed[i] < v;
+      for (int32_t i = s0; i < s1; ++i) r += placed[i] < v;
+    }
   }
   sorted[r] = v;
   rank[v] = r;

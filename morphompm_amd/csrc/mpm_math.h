@@ -396,6 +396,10 @@ MPM_HD int cell_base(T x, T inv_dx, T& fx) {
 #pragma clang fp contract(off)
 #endif
   const T X = x * inv_dx;
+  if (!(std::fabs(X) < T(1 << 20))) {  // NaN, inf or far off: no grid holds it
+    fx = T(1);
+    return -(1 << 24);
+  }
   const T b = std::floor(X - T(0.5));
   fx = X - b;
   return (int)b;

@@ -148,15 +148,18 @@ def bioink_blob(cells=64, N=128, seed=0, n_steps=300, dt=1e-4, tau_y=50.0, K=10.
     """Rate-dependent bioink: 128^3 grid, dx = 1/128, centred 64^3-cell blob at
     8 ppc -> 2,097,152 particles; Herschel-Bulkley tau_y = 50, K = 10,
     n = 0.6, m = 100, eps = 1e-4 (reference defaults), with kappa = the bulk
-    modulus lambda + 2 mu / 3 of the E = 1e3, nu = 0.3 material; radially graded
-    static growth g(r) = 1 + 0.1 (1 - r), r = distance from the centre / blob
-    half-width clipped to [0, 1]; 300 steps, checkpoint every 25."""
+    modulus lambda + 2 mu / 3 of the reference's default material (E = 1e4,
+    nu = 0.3, rho = 1000, config.py:7-11); radially graded static growth
+    g(r) = 1 + 0.1 (1 - r), r = distance from the centre / blob half-width
+    clipped to [0, 1]; dt = 1e-4, 300 steps, checkpoint every 25.  (At
+    rho = 1 the regularised viscosity tau_y m ~ 5e3 makes explicit steps at
+    this dx diverge within a few steps.)"""
     from . import constitutive as Km
     rng = np.random.default_rng(seed)
     dx = 1.0 / N
     lo = (N - cells) // 2
     pts = lattice(_box_cells([lo] * 3, [lo + cells] * 3), dx, 8, rng)
-    mat = Material(E=1.0e3, nu=0.3, density=1.0)
+    mat = Material(E=1.0e4, nu=0.3, density=1000.0)
     kappa = mat.lam + 2.0 * mat.mu / 3.0
     cfg = SimConfig(N=N, dx=dx, dt=dt, damping=0.05, sp=dx / 2, material=mat)
     model = Km.HerschelBulkley(kappa=kappa, K=K, n=n, tau_y=tau_y, m=100.0, eps=1e-4)

@@ -240,3 +240,104 @@ def test_bilayer_bending_curvature_vs_reference_fixture():
         assert k < k_t
         ks.append(k)
     assert abs(ks[1] / ks[0] - fx[1, 1] / fx[0, 1]) <= 0.02 * fx[1, 1] / fx[0, 1]
+
+
+# --------------------------------------------------------- full-size workloads
+def _timed(fn):
+    import time
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t
+
+
+def test_bioink_full_size_theta_gradient():
+    """The full bioink workload: 2,097,152 particles on 128^3, Herschel-Bulkley,
+    300 steps, checkpoint every 25.  (tau_y, K) gradient: FP64 adjoint vs
+    central FD of the FP64 forward (rel 1e-5); FP32 vs FP64 within 2e-2
+    (measured 8e-3 for tau_y, 1.4e-3 for K).  The FP32 bound is loose
+    because this gradient is not stable to perturbations of the trajectory at
+    that level: the FP64 loss's own central differences scatter by 6e-4 at
+    h = 1e-3, and FP32 rounding shifts when particles cross cells the same
+    way (at the reduced sizes, and for the other workloads, FP32 meets 1e-5).
+    The loss is only C1 in theta (particles cross cells), so the FD error
+    grows with h from the jumps in the second derivative and with 1/h from
+    roundoff of a loss ~6e6: h = 1e-4 (tau_y = 50) and 1e-3 (K = 10) sit
+    between (measured FD vs adjoint over h = 1e-1 .. 1e-5: 9e-2, 6e-5, 6e-4,
+    2e-6, 6e-6 for tau_y; 3e-2, 2e-4, 1e-7, 1e-6, 2e-5 for K)."""
+    from morphompm_amd import autograd as AG
+    from morphompm_amd.scenes import bioink_blob
+    sc = bioink_blob()
+    assert sc.n == 2097152 and sc.cfg.shape == (128, 128, 128) and sc.n_steps == 300
+
+    def run(theta, dtype, grad=True):
+        th = torch.tensor(theta, dtype=dtype, device="cuda", requires_grad=grad)
+        with torch.set_grad_enabled(grad):
+            x, v, C, F = AG.rollout(*_rest(sc, dtype), torch.tensor(sc.G, dtype=dtype, device="cuda"),
+                                    sc.model, sc.cfg, sc.n_steps, advect=True,
+                                    checkpoint_every=25, hb_params=th)
+            L = (x.double() ** 2).sum() + torch.linalg.det(F.double()).sum()
+        if grad:
+            L.backward()
+            return float(L), th.grad.double().cpu().numpy()
+        return float(L), None
+    (L64, g64), t64 = _timed(lambda: run([50.0, 10.0], torch.float64))
+    print(f"bioink FP64 gradient {g64} in {t64:.2f} s", flush=True)
+    for k, h in ((0, 1e-4), (1, 1e-3)):
+        tp, tm = [50.0, 10.0], [50.0, 10.0]
+        tp[k] += h
+        tm[k] -= h
+        fd = (run(tp, torch.float64, False)[0] - run(tm, torch.float64, False)[0]) / (2 * h)
+        print(f"bioink theta[{k}]: adjoint {g64[k]:.10e} FD {fd:.10e}", flush=True)
+        assert abs(g64[k] - fd) <= 1e-5 * abs(fd) + 1e-12, (k, g64[k], fd)
+    (_, g32), t32 = _timed(lambda: run([50.0, 10.0], torch.float32))
+    rel = np.abs(g32 - g64) / np.abs(g64)
+    print(f"bioink 2.1M x 300: FP32 fwd+adj {sc.n * sc.n_steps / t32:.3e} particle-steps/s "
+          f"({t32:.2f} s), FP64 {t64:.2f} s; grad rel.err FP32 {rel}")
+    assert np.all(rel <= 2e-2), (g32, g64)
+
+
+def test_shell_full_size_control_grid_fd32():
+    """The full shell workload: ~0.91 M particles on 100^3, Hencky, 300 steps,
+    anisotropic growth from the 16^3 x 3 control grid (12,288 parameters).
+    Gradient w.r.t. all parameters: along 32 seeded random unit directions vs
+    central FD of the FP64 forward (rel 1e-5); FP32 vs FP64 (rel 1e-5)."""
+    from morphompm_amd import autograd as AG
+    from morphompm_amd.growth import ControlGridGrowth, position_misfit
+    from morphompm_amd.scenes import shell
+    sc = shell()
+    assert 900_000 < sc.n < 1_000_000 and sc.cfg.shape == (100, 100, 100) and sc.n_steps == 300
+    ctrl = ControlGridGrowth(sc.pts, sc.extra["ctrl"])
+    theta0 = torch.tensor(sc.extra["theta"], device="cuda")
+    target = torch.tensor(sc.pts, device="cuda") * 1.01
+
+    def run(theta, dtype, grad=True):
+        th = theta.to(dtype).detach().requires_grad_(grad)
+        with torch.set_grad_enabled(grad):
+            x, v, C, F = AG.rollout(*_rest(sc, dtype), ctrl.fg(th), sc.model, sc.cfg, sc.n_steps,
+                                    advect=True, checkpoint_every=25)
+            L = position_misfit(x.double(), target)
+        if grad:
+            L.backward()
+            return float(L), th.grad.double()
+        return float(L), None
+    (_, g64), t64 = _timed(lambda: run(theta0, torch.float64))
+    print(f"shell FP64 gradient in {t64:.2f} s", flush=True)
+    rng = np.random.default_rng(0)
+    h = 1e-7
+    worst = 0.0
+    for _ in range(32):
+        d = torch.tensor(rng.standard_normal(theta0.shape), device="cuda")
+        d /= d.norm()
+        fd = (run(theta0 + h * d, torch.float64, False)[0]
+              - run(theta0 - h * d, torch.float64, False)[0]) / (2 * h)
+        an = float((g64 * d).sum())
+        worst = max(worst, abs(an - fd) / abs(fd))
+        print(f"shell direction: adjoint {an:.10e} FD {fd:.10e}", flush=True)
+        assert abs(an - fd) <= 1e-5 * abs(fd) + 1e-13, (an, fd)
+    (_, g32), t32 = _timed(lambda: run(theta0, torch.float32))
+    rel = float((g32 - g64).norm() / g64.norm())
+    print(f"shell {sc.n} x 300: FP32 fwd+adj {sc.n * sc.n_steps / t32:.3e} particle-steps/s "
+          f"({t32:.2f} s), FP64 {t64:.2f} s; FD worst rel {worst:.2e}; FP32 grad rel {rel:.2e}")
+    assert rel <= 1e-5
