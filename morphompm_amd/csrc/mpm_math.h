@@ -238,11 +238,17 @@ template <typename D, typename S> MPM_HD M3<D> cast3(const M3<S>& A) {
 
 // FP32 near-repeated branch: when the smallest eigenvalue gap of b is below
 // kNearRepeated times tr b, the FP32 path redoes the eigen-decomposition and
-// the divided differences in FP64.
+// the divided differences in FP64.  Above it FP32 is accurate: the FP32
+// eigenvectors err by ~eps/gap, but U (K o L) U^T depends on them only
+// through the variation of L across the near-degenerate block, which is
+// ~gap, so the product errs by ~eps |M| for any gap the FP32 Jacobi sweep
+// resolves.  The threshold sits at that resolution; a wider one sends whole
+// waves (any lane in the branch) down the FP64 path under near-isotropic
+// growth.
 #ifndef MPM_HENCKY_FP64_DEGEN
 #define MPM_HENCKY_FP64_DEGEN 1
 #endif
-constexpr float kNearRepeated = 1.0e-3f;
+constexpr float kNearRepeated = 1.0e-6f;
 
 MPM_HD bool near_repeated(const float beta[3]) {
   const float s = std::fabs(beta[0]) + std::fabs(beta[1]) + std::fabs(beta[2]);

@@ -46,10 +46,12 @@ class _CtrlToDiag(torch.autograd.Function):
     def backward(ctx, gdiag):
         g = ctx.grid
         contrib = (g.W[:, :, None] * gdiag.to(torch.float64)[:, None, :]).reshape(-1, 3)
-        srt = contrib[g.order]
-        cs = torch.cat([srt.new_zeros(1, 3), torch.cumsum(srt, 0)])
+        # prefix sums along the contiguous (innermost) axis: a scan down the
+        # 3-column axis of a tall array runs nearly serially on the device
+        srt = contrib[g.order].T.contiguous()                                  # (3, 8 n)
+        cs = torch.cat([srt.new_zeros(3, 1), torch.cumsum(srt, 1)], 1)
         out = torch.zeros(g.n_ctrl, 3, dtype=torch.float64, device=gdiag.device)
-        out[g.nodes] = cs[g.ends] - cs[g.starts]
+        out[g.nodes] = (cs[:, g.ends] - cs[:, g.starts]).T
         return out.reshape(g.shape).to(gdiag.dtype), None
 
 
