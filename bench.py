@@ -44,7 +44,9 @@ KERNELS = {
     7: ("grid_adj", lambda n, a, nb: 48 * a),               # partials + grid in, adjoint grid out
     8: ("adj_gather", lambda n, a, nb: 204 * n + 32 * a),   # sorted, 2 payloads, xn_b, nodes in;
                                                              # x_b v_b MD out
-    9: ("adj_vjp", lambda n, a, nb: 360 * n),               # MD C F G Cn Fn_b G_b in, C_b F_b G_b out
+    9: ("adj_vjp", lambda n, a, nb: 504 * n),               # MD C F G Cn Fn_b G_b, previous
+                                                             # x F, x_b v_b in; C_b F_b G_b and
+                                                             # the previous adjoint payload out
 }
 ADJ_KERNELS = {5, 6, 7, 8, 9}
 
