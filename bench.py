@@ -238,8 +238,13 @@ def main():
         # the timer is armed for exactly one iteration: the sum is its share
         ktab[kname] = {"id": kid, "avg_ms": float(np.mean(d)) if d else 0.0,
                        "launches_timed": cnt, "ms_per_iteration": float(np.sum(d))}
-    grids = bi.traj.seg_grid
-    active = float(np.mean([int((grids[k][:, 0] > 1e-12).sum()) for k in range(grids.shape[0])]))
+    # active nodes: one more step of the final state into a zeroed grid record
+    # (step records are written only where particles reach)
+    p = bi.plan
+    g0, sb0, pl0, d1 = p.new_grid(torch.float32), p.new_sort(), p.new_payload(torch.float32), p.new_dyn(torch.float32)
+    p.step_fwd(bi.final_state(), bi.G, None, d1, g0, sb0, pl0, True)
+    active = float(int((g0[:, 0] > 1e-12).sum()))
+    del g0, sb0, pl0, d1
     # occupied bins of this step's sort: n_active units (last word of a sort record)
     occ_bins = float(np.mean([int(bi.traj.seg_sort[k][-1]) for k in range(bi.traj.seg_sort.shape[0])])
                      ) * int(np.prod(bi.plan.tile))
