@@ -29,7 +29,8 @@
  *   particle (fx[3], q[3], A'[9], cell) -- the forward step's momentum map
  *   m_i += w pmass, mom_i += w (q + A' o) for stencil offset o, and the packed
  *   cell ((base+2) per axis in 10 bits, all ones off the grid).  Grids are
- *   limited to 1021 nodes per axis by this packing.
+ *   limited to 1022 nodes per axis by this packing (base+2 <= 1023); the
+ *   Python plan rejects larger grids before any launch.
  *
  * Reference interfaces these replace (paths in 2nugu/morphompm):
  *   python/morphompm/constitutive.py:1-12 (stress / stress_vjp seam),

@@ -1262,7 +1262,6 @@ int validate(const mpm_batch* b) {
     return set_err(MPM_EINVAL, "n_bins must be n_tiles * tile cells");
   if (b->block != 64 && b->block != 128 && b->block != 256)
     return set_err(MPM_EINVAL, "block must be 64, 128 or 256 (got %d)", b->block);
-  if (b->n_inst > 0 && b->inst == nullptr) return set_err(MPM_EINVAL, "batch tables missing");
   if (tile_index(b) < 0)
     return set_err(MPM_ENOTSUP, "tile shape %dx%dx%d not built", b->tile[0], b->tile[1], b->tile[2]);
   if (b->model_chunk != nullptr) {

@@ -25,6 +25,24 @@ from ._lib import MpmBatch, MpmInstance, SLOT_ALIGN, check, ptr, stream_ptr, suf
 N_DYN = 24        # x(3) v(3) C(9) F(9)
 ROW_X, ROW_V, ROW_C, ROW_F = 0, 3, 6, 15
 DEFAULT_TILE = (4, 8, 8)
+# The payload packs a particle's cell (base + 2 per axis, 0 .. nodes + 1) in
+# 10 bits per axis, so an instance grid holds at most 1022 nodes per axis.
+MAX_AXIS_NODES = 1022
+
+
+def check_spec(s, i=0):
+    """ValueError for an instance the kernels cannot run: a grid axis outside
+    1..MAX_AXIS_NODES, a negative particle count, or a non-positive or
+    non-finite dx / dt / particle mass / volume."""
+    if len(s.shape) != 3 or any(int(d) < 1 or int(d) > MAX_AXIS_NODES for d in s.shape):
+        raise ValueError(f"instance {i}: grid {tuple(s.shape)} must have 1..{MAX_AXIS_NODES} "
+                         "nodes per axis")
+    if int(s.n) < 0:
+        raise ValueError(f"instance {i}: particle count {s.n} is negative")
+    for name in ("dx", "dt", "pmass", "vol0"):
+        v = float(getattr(s, name))
+        if not (np.isfinite(v) and v > 0.0):
+            raise ValueError(f"instance {i}: {name} = {v} must be positive and finite")
 
 
 def model_spec(model):
@@ -81,8 +99,12 @@ def _ceil(a, b):
 
 class BatchPlan:
     def __init__(self, specs, tile=DEFAULT_TILE, block=256, device="cuda"):
-        _lib.require_gpu()
         self.specs = list(specs)
+        if not self.specs:
+            raise ValueError("a batch needs at least one instance")
+        for i, s in enumerate(self.specs):
+            check_spec(s, i)
+        _lib.require_gpu()
         self.tile = tuple(int(t) for t in tile)
         self.block = int(block)
         self.device = torch.device(device)

@@ -10,7 +10,7 @@ pairs laid out particle-major, i.e. the same per-node accumulation order as the
 reference's loops.
 
 Extensions beyond the reference (each pinned by finite differences and by
-reduction to the reference semantics in tests/test_oracle_vec.py):
+reduction to the reference semantics in tests/test_oracle.py):
   * non-cubic grids ``cfg.dims = (Nx, Ny, Nz)``;
   * growth schedules: before step t, Fg_t = I + a_t (G - I) with G the state's
     Fg at t = 0 (a_t == 1 keeps G exactly);  G_bar = sum_t a_t Fg_bar_t;
