@@ -116,15 +116,19 @@ class BatchedInverse:
         self._g_fwd = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self._g_fwd):
             self._fwd()
+        self._seg_after_fwd = self.traj._loaded
         self._g_bwd = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self._g_bwd):
             self._bwd()
+        self._seg_after_bwd = self.traj._loaded
 
     def run_forward(self):
         if self.use_graphs:
             if self._g_fwd is None:
                 self._capture()
             self._g_fwd.replay()
+            # a replay leaves the segment store as the captured run did
+            self.traj._loaded = self._seg_after_fwd
         else:
             self._fwd()
         return self._final
@@ -134,6 +138,7 @@ class BatchedInverse:
             if self._g_bwd is None:
                 self._capture()
             self._g_bwd.replay()
+            self.traj._loaded = self._seg_after_bwd
         else:
             self._bwd()
         return self.traj.G_bar

@@ -1,11 +1,12 @@
 // MLS-MPM growth step and its hand-written adjoint for gfx950 (MI355X).
 //
 // Per forward step (transfer.step, python/morphompm/transfer.py:33-79):
-//   sort     : bin key per particle -> counting sort (integer atomics only for
-//              the counts) -> per-bin sort by slot, so the order inside every
-//              bin is fixed;
-//   prep     : stress, grown volume and the APIC payload of each particle,
-//              written in sorted order;
+//   prep     : stress, grown volume and the APIC payload of each particle
+//              (slot order), its bin key and the bin counts -- done by the
+//              previous step's G2P when steps are chained;
+//   sort     : counting sort over this step's occupied bins (integer atomics
+//              only, for counts and places in a bin) -> per-bin sort by
+//              slot, so the order inside every bin is fixed;
 //   p2g tile : one thread per bin sums its particles' 27 stencil contributions
 //              in registers, then adds them into an LDS node tile in 27
 //              barrier-separated rounds (one writer per node per round): a
@@ -273,60 +274,75 @@ __global__ __launch_bounds__(256) void k_unit_rank(const int32_t* unit_inst, int
     *n_active = base + wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-// Turns the unit-level keys (unit * TB + place) into compact bin keys of
-// this step's occupied units, in place, and counts the bins.
-// A key outside the flagged units (only possible when a chained step is
-// told its keys are ready and they are not) goes to the dead bin: wrong, but
-// every index stays in bounds.
-__global__ __launch_bounds__(256) void k_sort_key(const mpm_instance* inst, const int32_t* slot_inst,
-                                                  int64_t n_slots, int TB, int64_t n_units,
-                                                  const int32_t* unit_rank, int32_t* key,
-                                                  int32_t* slot, int32_t* count) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const mpm_instance& I = inst[block_inst(slot_inst)];
-  // The dead unit's bin 0 holds live particles with no in-grid node and its
-  // bin 1 the padding slots, so each instance's sorted range is exactly its
-  // own slot range [p_off, p_off + padded count): padding slots take the tail
-  // of that range in slot order (k_place) and no atomic slot; the first
-  // padding slot sets bin 1's count once.
-  int64_t k = -1;
-  if (p < n_slots) {
-    const int64_t dead = (int64_t)unit_rank[dead_unit(I)] * TB;
-    const int64_t local = p - I.p_off;
-    if (local == I.n) {
-      const int64_t padded = ((int64_t)I.n + kChunk - 1) / kChunk * kChunk;
-      atomicAdd(&count[dead + 1], (int32_t)((padded > 0 ? padded : kChunk) - I.n));
-    }
-    if (local < I.n) {
-      const int32_t raw = key[p];
-      const int32_t ur = (raw >= 0 && raw / TB < n_units) ? unit_rank[raw / TB] : -1;
-      k = ur >= 0 ? (int64_t)ur * TB + raw % TB : dead;
-    }
-  }
-  // One atomic per run of equal keys in adjacent lanes (particles arrive
-  // nearly sorted); the per-bin sort afterwards fixes the order in a bin.
+// Bin counts are kept per unit-level key (unit * TB + place) by the kernels
+// that compute the keys (no-return atomics: the producer never waits on
+// them); the scan reads the counts of this step's occupied units only.  A
+// wave adds one atomic per run of equal keys in adjacent lanes (particles
+// arrive nearly sorted).  Every lane of the wave must call these; k < 0
+// counts nothing.
+__device__ __forceinline__ void run_heads(int32_t k, bool& head, int& my_head, int& run) {
   const int lane = threadIdx.x & 63;
-  const int32_t k32 = (int32_t)k;
-  const int32_t prev = __shfl_up(k32, 1);
-  const bool head = (lane == 0) || (prev != k32);
+  const int32_t prev = __shfl_up(k, 1);
+  head = (lane == 0) || (prev != k);
   const unsigned long long heads = __ballot(head);
   const unsigned long long upto = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
-  const int my_head = 63 - __clzll(upto);
+  my_head = 63 - __clzll(upto);
   const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1));
-  const int next_head = after ? lane + __ffsll(after) : 64;
-  int32_t base_slot = 0;
-  if (head && k32 >= 0) base_slot = atomicAdd(&count[k32], next_head - lane);
-  base_slot = __shfl(base_slot, my_head);
-  if (p < n_slots) {
-    key[p] = k32;
-    if (k32 >= 0) slot[p] = base_slot + (lane - my_head);
-  }
+  run = (after ? lane + __ffsll(after) : 64) - lane;
 }
 
-// Exclusive scan of count[n] -> start[n + 1] over the n = n_active * TB bins
-// of this step, in three passes; the counts are cleared for the next step.
-__global__ void k_scan_blocks(int32_t* count, int32_t* start, int32_t* bsum,
-                              const int32_t* n_active, int TB) {
+__device__ __forceinline__ void count_keys(int32_t k, int32_t* count) {
+  bool head;
+  int my_head, run;
+  run_heads(k, head, my_head, run);
+  if (head && k >= 0) atomicAdd(&count[k], run);
+}
+
+// The same runs on a cursor, returning each lane's place among its bin's
+// particles (k_bin_rank fixes the order in a bin).
+__device__ __forceinline__ int32_t place_in_bin(int32_t k, int32_t* cursor) {
+  bool head;
+  int my_head, run;
+  run_heads(k, head, my_head, run);
+  int32_t base_slot = 0;
+  if (head && k >= 0) base_slot = atomicAdd(&cursor[k], run);
+  base_slot = __shfl(base_slot, my_head);
+  return base_slot + ((int)(threadIdx.x & 63) - my_head);
+}
+
+// Clears the bin counts and unit flags of the whole batch (before keys are
+// computed from scratch: a chained G2P whose counts no sort consumed may have
+// left some behind).
+__global__ __launch_bounds__(256) void k_clear_bins(int32_t* count, int64_t n_bins,
+                                                    int32_t* unit_flag, int64_t n_units) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n_bins) count[i] = 0;
+  if (i < n_units) unit_flag[i] = 0;
+}
+
+// The dead unit's bin 0 holds live particles with no in-grid node and its bin
+// 1 the padding slots, so each instance's sorted range is exactly its own
+// slot range [p_off, p_off + padded count): padding slots keep key -1, take
+// the tail of that range in slot order (k_place) and no place from the
+// cursor; the first padding slot (local == n) sets bin 1's count once.
+__device__ __forceinline__ void count_padding(const mpm_instance& I, int64_t local, int TB,
+                                              int32_t* count) {
+  if (local != I.n) return;
+  const int64_t padded = ((int64_t)I.n + kChunk - 1) / kChunk * kChunk;
+  atomicAdd(&count[dead_unit(I) * TB + 1], (int32_t)((padded > 0 ? padded : kChunk) - I.n));
+}
+
+// Stores the unit-level key of slot p and counts it.
+__device__ __forceinline__ void put_key(int64_t p, int32_t k, int32_t* key, int32_t* count) {
+  count_keys(k, count);
+  key[p] = k;
+}
+
+// Exclusive scan of the counts of this step's n = n_active * TB bins (bin
+// a * TB + l is place l of unit active_unit[a]) -> start[n + 1], in two
+// passes; the counts are cleared for the next step.
+__global__ void k_scan_blocks(int32_t* count, const int32_t* active_unit, int32_t* start,
+                              int32_t* cursor, int32_t* bsum, const int32_t* n_active, int TB) {
   __shared__ int32_t sh[kScanBlock];
   const int64_t n = (int64_t)__builtin_amdgcn_readfirstlane(*n_active) * TB;
   const int64_t i0 = ((int64_t)blockIdx.x * kScanBlock + threadIdx.x) * kScanPer;
@@ -335,9 +351,12 @@ __global__ void k_scan_blocks(int32_t* count, int32_t* start, int32_t* bsum,
   int32_t tot = 0;
   for (int k = 0; k < kScanPer; ++k) {
     v[k] = 0;
-    if (i0 + k < n) {
-      v[k] = count[i0 + k];
-      count[i0 + k] = 0;
+    const int64_t i = i0 + k;
+    if (i < n) {
+      const int64_t c = (int64_t)active_unit[i / TB] * TB + i % TB;
+      v[k] = count[c];
+      count[c] = 0;
+      cursor[i] = 0;
     }
     tot += v[k];
   }
@@ -372,17 +391,32 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_add(int32_t* start, const i
   if ((blk + 1) * kScanBlock * kScanPer >= n && threadIdx.x == 0) start[n] = base + bsum[blk];
 }
 
-// Bins in arbitrary order within each bin: placed[start[k] + slot[p]] = p.
-__global__ void k_place(const int32_t* key, const int32_t* slot, const int32_t* start,
-                        int32_t* placed, int64_t n_slots) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_slots) return;
-  const int32_t k = key[p];
-  if (k < 0) {
+// Bins in arbitrary order within each bin: placed[start[k] + place] = p,
+// with k the compact key unit_rank[unit] * TB + place in tile, written back
+// to key, and the place in the bin from an atomic cursor.  A key outside the
+// occupied units (only possible when a chained step is told its keys are
+// ready and they are not) is dropped: wrong, but every index stays in bounds.
+// (Whole waves only leave early: place_in_bin needs every lane.)
+__global__ void k_place(const int32_t* unit_rank, int TB, int64_t n_units, int32_t* key,
+                        const int32_t* start, int32_t* cursor, int32_t* placed,
+                        int64_t n_slots) {
+  const int64_t p0 = (int64_t)blockIdx.x * blockDim.x, p = p0 + threadIdx.x;
+  if (p0 >= n_slots) return;
+  const int32_t raw = key[p];
+  int32_t k = -1;
+  if (raw >= 0) {
+    const int32_t ur = raw / TB < n_units ? unit_rank[raw / TB] : -1;
+    if (ur >= 0) k = ur * TB + raw % TB;
+  }
+  const int32_t place = place_in_bin(k, cursor);
+  if (raw < 0) {
     placed[p] = (int32_t)p;  // padding slot: the tail of its instance's range, in order
     return;
   }
-  placed[start[k] + slot[p]] = (int32_t)p;
+  key[p] = k;
+  if (k < 0) return;
+  const int64_t pos = (int64_t)start[k] + place;
+  if (pos >= 0 && pos < n_slots) placed[pos] = (int32_t)p;
 }
 
 // Stable order within each bin: the particle at placed position j goes to
@@ -475,45 +509,60 @@ __device__ __forceinline__ void make_payload(const Inst<T>& I, const T x[3], con
   store_payload(rec, fx, q, Ap, pack_cell(base, I.nx, I.ny, I.nz));
 }
 
-// Payload, sort key and unit flag of a step from its input state.
+// Payload, sort key, bin count and unit flag of a step from its input state.
+// (Whole waves only leave early: count_in_bin needs every lane.)
 template <typename T, int MODEL>
 __global__ __launch_bounds__(256) void k_prep_fwd(const mpm_instance* inst,
                                                   const int32_t* slot_inst, int64_t n_slots,
                                                   const T* dyn, const T* Fgp, const T* sched,
                                                   int TX, int TY, int TZ, T* payload, int32_t* key,
-                                                  int32_t* unit_flag, const int32_t* chunks) {
+                                                  int32_t* count, int32_t* unit_flag,
+                                                  const int32_t* chunks) {
   const int64_t p0 = block_first_slot(chunks), p = p0 + threadIdx.x;
-  if (p >= n_slots) return;
+  if (p0 >= n_slots) return;
   const int ii = __builtin_amdgcn_readfirstlane(slot_inst[p0 / kChunk]);
   const Inst<T> I = load_inst<T, MODEL>(inst, ii);
-  if (p - I.p_off >= I.n) return;
-  T x[3], v[3];
-  ld3(dyn, n_slots, p, RX, x);
-  ld3(dyn, n_slots, p, RV, v);
-  const M3<T> C = ld9(dyn, n_slots, p, RC);
-  const M3<T> F = ld9(dyn, n_slots, p, RF);
-  const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), sched ? sched[ii] : T(1));
-  int base[3];
-  make_payload(I, x, v, C, F, Fg, payload + p * kRecord, base);
-  key[p] = cell_key(inst[ii], base, TX, TY, TZ, unit_flag);
+  const int64_t local = p - I.p_off;
+  int32_t k = -1;
+  if (local < I.n) {
+    T x[3], v[3];
+    ld3(dyn, n_slots, p, RX, x);
+    ld3(dyn, n_slots, p, RV, v);
+    const M3<T> C = ld9(dyn, n_slots, p, RC);
+    const M3<T> F = ld9(dyn, n_slots, p, RF);
+    const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), sched ? sched[ii] : T(1));
+    int base[3];
+    make_payload(I, x, v, C, F, Fg, payload + p * kRecord, base);
+    k = cell_key(inst[ii], base, TX, TY, TZ, unit_flag);
+  } else {
+    count_padding(inst[ii], local, TX * TY * TZ, count);
+  }
+  put_key(p, k, key, count);
 }
 
-// Sort keys and unit flags from positions only (mpm_sort_by_cell).
+// Sort keys, bin counts and unit flags from positions only (mpm_sort_by_cell).
 template <typename T>
 __global__ __launch_bounds__(256) void k_cell_keys(const mpm_instance* inst,
                                                    const int32_t* slot_inst, int64_t n_slots,
                                                    const T* dyn, int TX, int TY, int TZ,
-                                                   int32_t* key, int32_t* unit_flag) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_slots) return;
+                                                   int32_t* key, int32_t* count,
+                                                   int32_t* unit_flag) {
+  const int64_t p0 = (int64_t)blockIdx.x * blockDim.x, p = p0 + threadIdx.x;
+  if (p0 >= n_slots) return;
   const int ii = block_inst(slot_inst);
   const Inst<T> I = load_inst<T>(inst, ii);
-  if (p - I.p_off >= I.n) return;
-  T x[3], fx[3];
-  int base[3];
-  ld3(dyn, n_slots, p, RX, x);
-  stencil_base(x, I, base, fx);
-  key[p] = cell_key(inst[ii], base, TX, TY, TZ, unit_flag);
+  const int64_t local = p - I.p_off;
+  int32_t k = -1;
+  if (local < I.n) {
+    T x[3], fx[3];
+    int base[3];
+    ld3(dyn, n_slots, p, RX, x);
+    stencil_base(x, I, base, fx);
+    k = cell_key(inst[ii], base, TX, TY, TZ, unit_flag);
+  } else {
+    count_padding(inst[ii], local, TX * TY * TZ, count);
+  }
+  put_key(p, k, key, count);
 }
 
 // ---------------------------------------------------------------------------
@@ -779,84 +828,92 @@ __device__ __forceinline__ GridNodes<T> grid_nodes(const T* rec, const Inst<T>& 
 // ---------------------------------------------------------------------------
 // G2P (transfer.py:61-77), one thread per particle slot (coalesced state
 // reads and writes); stencil nodes come from the grid through the caches.
-// NEXT: also writes the next step's payload, sort key and unit flag from the
-// new state (what k_prep_fwd would compute from dyn_out, bit for bit), with
-// the next step's growth factor sched_next.
+// NEXT: also writes the next step's payload, sort key, bin count and unit
+// flag from the new state (what k_prep_fwd would compute from dyn_out, bit for
+// bit), with the next step's growth factor sched_next.
 // ---------------------------------------------------------------------------
 template <typename T, int MODEL, bool NEXT>
 __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int32_t* slot_inst,
                                              int64_t n_slots, const T* dyn_in, const T* grid,
                                              T* dyn_out, int advect, const T* Fgp,
                                              const T* sched_next, int TX, int TY, int TZ,
-                                             T* payload_next, int32_t* key, int32_t* unit_flag,
-                                             const int32_t* chunks) {
+                                             T* payload_next, int32_t* key, int32_t* count,
+                                             int32_t* unit_flag, const int32_t* chunks) {
   const int64_t p0 = block_first_slot(chunks), p = p0 + threadIdx.x;
-  if (p >= n_slots) return;
+  if (p0 >= n_slots) return;
   const int ii = __builtin_amdgcn_readfirstlane(slot_inst[p0 / kChunk]);
   const Inst<T> I = load_inst<T, MODEL>(inst, ii);
-  if (p - I.p_off >= I.n) return;
-  T x[3], fx[3];
-  int base[3];
-  ld3(dyn_in, n_slots, p, RX, x);
-  stencil_base(x, I, base, fx);
-  const bool valid = base_in_grid(base, I.nx, I.ny, I.nz);
-  // Separable weights: with d = (o - fx) dx,
-  //   nv = sum w v,   nC = k4 dx (sum w v o^T - nv fx^T),
-  // and each z-row (i, j) of 3 nodes is summed first.
-  T nv[3] = {T(0), T(0), T(0)};
-  M3<T> nC = zero3<T>();
-  auto gather = [&](const auto& nodes) {
-    T w[3][3];
-    for (int c = 0; c < 3; ++c) bspline_w(fx[c], w[c]);
-    const T wz0 = w[2][0], wz1 = w[2][1], wz2 = w[2][2], wz2k = T(2) * w[2][2];
-    T Si[3] = {T(0), T(0), T(0)}, Sj[3] = {T(0), T(0), T(0)}, Sk[3] = {T(0), T(0), T(0)};
-    // x planes stay rolled to bound the gathered values in flight
+  const int64_t local = p - I.p_off;
+  if (!NEXT && local >= I.n) return;
+  // (with NEXT, whole waves only leave early: count_in_bin needs every lane)
+  int32_t next_key = -1;
+  if (local < I.n) {
+    T x[3], fx[3];
+    int base[3];
+    ld3(dyn_in, n_slots, p, RX, x);
+    stencil_base(x, I, base, fx);
+    const bool valid = base_in_grid(base, I.nx, I.ny, I.nz);
+    // Separable weights: with d = (o - fx) dx,
+    //   nv = sum w v,   nC = k4 dx (sum w v o^T - nv fx^T),
+    // and each z-row (i, j) of 3 nodes is summed first.
+    T nv[3] = {T(0), T(0), T(0)};
+    M3<T> nC = zero3<T>();
+    auto gather = [&](const auto& nodes) {
+      T w[3][3];
+      for (int c = 0; c < 3; ++c) bspline_w(fx[c], w[c]);
+      const T wz0 = w[2][0], wz1 = w[2][1], wz2 = w[2][2], wz2k = T(2) * w[2][2];
+      T Si[3] = {T(0), T(0), T(0)}, Sj[3] = {T(0), T(0), T(0)}, Sk[3] = {T(0), T(0), T(0)};
+      // x planes stay rolled to bound the gathered values in flight
 #pragma unroll 1
-    for (int i = 0; i < 3; ++i) {
-      const T wxi = i == 0 ? w[0][0] : (i == 1 ? w[0][1] : w[0][2]);
-      const T Ti = T(i);
+      for (int i = 0; i < 3; ++i) {
+        const T wxi = i == 0 ? w[0][0] : (i == 1 ? w[0][1] : w[0][2]);
+        const T Ti = T(i);
 #pragma unroll
-      for (int jj = 0; jj < 3; ++jj) {
-        const V4<T> g0 = nodes(i, jj, 0), g1 = nodes(i, jj, 1), g2 = nodes(i, jj, 2);
-        const T v0[3] = {g0.y, g0.z, g0.w}, v1[3] = {g1.y, g1.z, g1.w},
-                v2[3] = {g2.y, g2.z, g2.w};
-        const T wxy = wxi * w[1][jj];
-        for (int c = 0; c < 3; ++c) {
-          const T r0 = wz0 * v0[c] + wz1 * v1[c] + wz2 * v2[c];
-          const T r1 = wz1 * v1[c] + wz2k * v2[c];
-          nv[c] += wxy * r0;
-          Sk[c] += wxy * r1;
-          Si[c] += (Ti * wxy) * r0;
-          if (jj) Sj[c] += (T(jj) * wxy) * r0;
+        for (int jj = 0; jj < 3; ++jj) {
+          const V4<T> g0 = nodes(i, jj, 0), g1 = nodes(i, jj, 1), g2 = nodes(i, jj, 2);
+          const T v0[3] = {g0.y, g0.z, g0.w}, v1[3] = {g1.y, g1.z, g1.w},
+                  v2[3] = {g2.y, g2.z, g2.w};
+          const T wxy = wxi * w[1][jj];
+          for (int c = 0; c < 3; ++c) {
+            const T r0 = wz0 * v0[c] + wz1 * v1[c] + wz2 * v2[c];
+            const T r1 = wz1 * v1[c] + wz2k * v2[c];
+            nv[c] += wxy * r0;
+            Sk[c] += wxy * r1;
+            Si[c] += (Ti * wxy) * r0;
+            if (jj) Sj[c] += (T(jj) * wxy) * r0;
+          }
         }
       }
+      const T s = I.k4 * I.dx;
+      for (int c = 0; c < 3; ++c) {
+        nC(c, 0) = s * (Si[c] - nv[c] * fx[0]);
+        nC(c, 1) = s * (Sj[c] - nv[c] * fx[1]);
+        nC(c, 2) = s * (Sk[c] - nv[c] * fx[2]);
+      }
+    };
+    if (valid) gather(grid_nodes(grid, I, base));
+    const M3<T> F = ld9(dyn_in, n_slots, p, RF);
+    // F_new = (I + dt C) F, formed as F + dt (C F): I + dt C would round away
+    // the low bits of the small increment the same way for every particle.
+    const M3<T> CF = mul(nC, F);
+    M3<T> Fn;
+    for (int k = 0; k < 9; ++k) Fn.a[k] = F.a[k] + I.dt * CF.a[k];
+    T xn[3];
+    for (int c = 0; c < 3; ++c) xn[c] = advect ? x[c] + I.dt * nv[c] : x[c];
+    st3(dyn_out, n_slots, p, RX, xn);
+    st3(dyn_out, n_slots, p, RV, nv);
+    st9(dyn_out, n_slots, p, RC, nC);
+    st9(dyn_out, n_slots, p, RF, Fn);
+    if (NEXT) {
+      const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), sched_next ? sched_next[ii] : T(1));
+      int nbase[3];
+      make_payload(I, xn, nv, nC, Fn, Fg, payload_next + p * kRecord, nbase);
+      next_key = cell_key(inst[ii], nbase, TX, TY, TZ, unit_flag);
     }
-    const T s = I.k4 * I.dx;
-    for (int c = 0; c < 3; ++c) {
-      nC(c, 0) = s * (Si[c] - nv[c] * fx[0]);
-      nC(c, 1) = s * (Sj[c] - nv[c] * fx[1]);
-      nC(c, 2) = s * (Sk[c] - nv[c] * fx[2]);
-    }
-  };
-  if (valid) gather(grid_nodes(grid, I, base));
-  const M3<T> F = ld9(dyn_in, n_slots, p, RF);
-  // F_new = (I + dt C) F, formed as F + dt (C F): I + dt C would round away
-  // the low bits of the small increment the same way for every particle.
-  const M3<T> CF = mul(nC, F);
-  M3<T> Fn;
-  for (int k = 0; k < 9; ++k) Fn.a[k] = F.a[k] + I.dt * CF.a[k];
-  T xn[3];
-  for (int c = 0; c < 3; ++c) xn[c] = advect ? x[c] + I.dt * nv[c] : x[c];
-  st3(dyn_out, n_slots, p, RX, xn);
-  st3(dyn_out, n_slots, p, RV, nv);
-  st9(dyn_out, n_slots, p, RC, nC);
-  st9(dyn_out, n_slots, p, RF, Fn);
-  if (NEXT) {
-    const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), sched_next ? sched_next[ii] : T(1));
-    int nbase[3];
-    make_payload(I, xn, nv, nC, Fn, Fg, payload_next + p * kRecord, nbase);
-    key[p] = cell_key(inst[ii], nbase, TX, TY, TZ, unit_flag);
+  } else if (NEXT) {
+    count_padding(inst[ii], local, TX * TY * TZ, count);
   }
+  if (NEXT) put_key(p, next_key, key, count);
 }
 
 // ---------------------------------------------------------------------------
@@ -1200,7 +1257,7 @@ int64_t tile_nodes(const mpm_batch* b) {
 inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
 struct Workspace {
-  int32_t *key, *slot, *placed, *count, *bsum, *unit_flag, *ucount;
+  int32_t *key, *cursor, *placed, *count, *bsum, *unit_flag, *ucount;
   void *payload, *partial, *gadj, *md;
 };
 
@@ -1233,7 +1290,7 @@ int64_t carve(const mpm_batch* b, size_t tsz, char* base, Workspace* w) {
   };
   Workspace ws;
   ws.key = (int32_t*)take(4 * b->n_slots);
-  ws.slot = (int32_t*)take(4 * b->n_slots);
+  ws.cursor = (int32_t*)take(4 * b->n_bins);
   ws.count = (int32_t*)take(4 * b->n_bins);
   ws.bsum = (int32_t*)take(4 * (scan_blocks(b->n_bins) + 1));
   ws.unit_flag = (int32_t*)take(4 * b->n_tiles);
@@ -1341,8 +1398,14 @@ struct TimedLaunch {
   }
 };
 
-// The sort from the unit-level keys and unit flags a prep, G2P or key kernel
-// left in the workspace.
+void clear_bins(const mpm_batch* b, const Workspace& w, hipStream_t st) {
+  const int64_t n = b->n_bins > b->n_tiles ? b->n_bins : b->n_tiles;
+  k_clear_bins<<<dim3((unsigned)((n + 255) / 256)), 256, 0, st>>>(w.count, b->n_bins,
+                                                                  w.unit_flag, b->n_tiles);
+}
+
+// The sort from the unit-level keys, bin counts and unit flags a prep, G2P or
+// key kernel left in the workspace.
 int sort_from_keys(const mpm_batch* b, int32_t* sortbuf, const Workspace& w, hipStream_t st) {
   const SortView<int32_t> v(b, sortbuf);
   const int bs = b->block;
@@ -1354,13 +1417,11 @@ int sort_from_keys(const mpm_batch* b, int32_t* sortbuf, const Workspace& w, hip
                                                    w.ucount);
   k_unit_rank<<<dim3((unsigned)nu), 256, 0, st>>>(b->tile_inst, b->n_tiles, w.unit_flag, w.ucount,
                                                   v.unit_rank, v.active_unit, v.n_active);
-  k_sort_key<<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, TB,
-                                                   b->n_tiles, v.unit_rank, w.key, w.slot,
-                                                   w.count);
-  k_scan_blocks<<<dim3((unsigned)nb), kScanBlock, 0, st>>>(w.count, v.start, w.bsum, v.n_active,
-                                                            TB);
+  k_scan_blocks<<<dim3((unsigned)nb), kScanBlock, 0, st>>>(w.count, v.active_unit, v.start,
+                                                            w.cursor, w.bsum, v.n_active, TB);
   k_scan_add<<<dim3((unsigned)nb), kScanBlock, 0, st>>>(v.start, w.bsum, v.n_active, TB);
-  k_place<<<grid1(b->n_slots, bs), bs, 0, st>>>(w.key, w.slot, v.start, w.placed, b->n_slots);
+  k_place<<<grid1(b->n_slots, bs), bs, 0, st>>>(v.unit_rank, TB, b->n_tiles, w.key, v.start,
+                                                w.cursor, w.placed, b->n_slots);
   k_bin_rank<<<grid1(b->n_slots, 256), 256, 0, st>>>(w.key, v.start, w.placed, v.sorted, v.rank,
                                                      b->n_slots);
   return check_launch("sort_by_cell");
@@ -1372,9 +1433,10 @@ int sort_impl(const mpm_batch* b, const T* dyn, int32_t* sortbuf, const Workspac
   const int bs = b->block;
   {
   TimedLaunch tl(MPM_K_SORT, st);
+  clear_bins(b, w, st);
   k_cell_keys<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn,
                                                         b->tile[0], b->tile[1], b->tile[2], w.key,
-                                                        w.unit_flag);
+                                                        w.count, w.unit_flag);
   }
   return sort_from_keys(b, sortbuf, w, st);
 }
@@ -1456,10 +1518,11 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   const int TX = b->tile[0], TY = b->tile[1], TZ = b->tile[2];
   if (!ready) {
     TimedLaunch tl(MPM_K_PREP_FWD, st);
+    clear_bins(b, w, st);
     per_model(b, bs, [&](auto M, const int32_t* ch, dim3 g) {
       k_prep_fwd<T, decltype(M)::value><<<g, bs, 0, st>>>(
           b->inst, b->slot_inst, b->n_slots, dyn_in, Fg, sched, TX, TY, TZ, payload, w.key,
-          w.unit_flag, ch);
+          w.count, w.unit_flag, ch);
     });
   }
   rc = sort_from_keys(b, sortbuf, w, st);
@@ -1471,12 +1534,12 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
     per_model(b, bs, [&](auto M, const int32_t* ch, dim3 g) {
       k_g2p<T, decltype(M)::value, true><<<g, bs, 0, st>>>(
           b->inst, b->slot_inst, b->n_slots, dyn_in, grid, dyn_out, advect, Fg, sched_next, TX,
-          TY, TZ, payload_next, w.key, w.unit_flag, ch);
+          TY, TZ, payload_next, w.key, w.count, w.unit_flag, ch);
     });
   else   // the G2P part does not depend on the model
     k_g2p<T, MPM_NEO_HOOKEAN, false><<<grid1(b->n_slots, bs), bs, 0, st>>>(
         b->inst, b->slot_inst, b->n_slots, dyn_in, grid, dyn_out, advect, nullptr, nullptr, TX,
-        TY, TZ, nullptr, nullptr, nullptr, nullptr);
+        TY, TZ, nullptr, nullptr, nullptr, nullptr, nullptr);
   return check_launch("step_fwd");
 }
 

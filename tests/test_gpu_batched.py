@@ -119,6 +119,21 @@ def test_graph_replays_match_eager():
         Lg, gg = graph.loss_and_grad()
         torch.cuda.synchronize()
         assert torch.equal(Le, Lg) and torch.equal(ge, gg), rep
+        # the final state after replays (the segment store was left holding
+        # the first segment by the replayed adjoint; an eager final_state()
+        # before the replays must not make it look current)
+        assert torch.equal(eager.final_state(), graph.final_state()), rep
+
+
+def test_final_state_after_eager_call_and_replays():
+    from morphompm_amd.batched import BatchedInverse
+    sc = _small_batch(("nh", "hencky"), n_steps=12)    # 3 segments of 5, 5, 2
+    graph = BatchedInverse(sc, dtype=torch.float32, use_graphs=True)
+    graph.set_theta([s.extra["rate"] for s in sc])
+    graph.loss_and_grad()
+    ref = graph.final_state().clone()                   # eager rerun of the last segment
+    graph.loss_and_grad()                               # replays: the store ends on segment 0
+    assert torch.equal(graph.final_state(), ref)
 
 
 def test_hb_theta_batched_fd_fp64():
