@@ -574,6 +574,7 @@ __global__ __launch_bounds__(256) void k_cell_keys(const mpm_instance* inst,
 // ---------------------------------------------------------------------------
 // INDIRECT: payload records in slot order, read through sorted[j]; else in
 // sorted order.
+constexpr int kTileIdxCap = 2048;   // staged slot indices per tile (8 KB of LDS)
 template <typename T, int TX, int TY, int TZ, bool MASS, bool INDIRECT>
 __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* inst,
                                                         const int32_t* unit_inst,
@@ -595,6 +596,16 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
   const int lz = tid % TZ, ly = (tid / TZ) % TY, lx = tid / (TY * TZ);
 
   for (int i = tid; i < 4 * TN; i += NT) lds[i] = T(0);
+  // INDIRECT: the tile's particles are the contiguous sorted range [r0, r1);
+  // their slot indices are staged in LDS with coalesced loads, so the record
+  // loads do not wait on a dependent index load each.
+  __shared__ int32_t sidx[INDIRECT ? kTileIdxCap : 1];
+  const int32_t r0 = bin_start[key0];
+  if (INDIRECT) {
+    const int32_t n_idx = min(bin_start[key0 + NT] - r0, kTileIdxCap);
+    for (int i = tid; i < n_idx; i += NT) sidx[i] = sorted[r0 + i];
+    __syncthreads();
+  }
 
   T aw[27], am[27][3];
 #pragma unroll
@@ -604,7 +615,8 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
   }
   const int32_t s0 = bin_start[key0 + tid], s1 = bin_start[key0 + tid + 1];
   for (int32_t j = s0; j < s1; ++j) {
-    const T* r = payload + (int64_t)(INDIRECT ? sorted[j] : j) * kRecord;
+    const int32_t src = INDIRECT ? (j - r0 < kTileIdxCap ? sidx[j - r0] : sorted[j]) : j;
+    const T* r = payload + (int64_t)src * kRecord;
     const V4<T> r0 = ld4(r), r1 = ld4(r + 4), r2 = ld4(r + 8), r3 = ld4(r + 12);
     // fx = r0.xyz, Q = (r0.w, r1.x, r1.y), A' row-major = r1.zw r2.xyzw r3.xyz
     const T q0 = r0.w, q1 = r1.x, q2 = r1.y;
