@@ -45,11 +45,16 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 4
+#define MPM_ABI_VERSION 5
 #define MPM_SLOT_ALIGN 256 /* particle and node ranges of an instance start at multiples of this */
 
 enum { MPM_OK = 0, MPM_EINVAL = -1, MPM_ELAUNCH = -2, MPM_ENOTSUP = -3 };
 enum { MPM_NEO_HOOKEAN = 0, MPM_HENCKY = 1, MPM_HERSCHEL_BULKLEY = 2 };
+/* Rows of dyn_out a chained step writes (mpm_step_fwd_chain_*): all of them
+ * (a checkpoint or a returned state), all but v (a step record: the reverse
+ * pass reads x, C and F of it), or x and F only (a state read by the next
+ * chained step alone).  Rows not written keep their previous contents. */
+enum { MPM_OUT_FULL = 0, MPM_OUT_RECORD = 1, MPM_OUT_NEXT = 2 };
 
 /* One independent simulation (python/morphompm/config.py:21-40 SimConfig plus
  * its constitutive model).  Node grid nx*ny*nz covers [0, n*dx) per axis.
@@ -160,18 +165,20 @@ int32_t mpm_step_fwd_f64(const mpm_batch* b, const double* dyn_in, const double*
  * the previous chained call (its payload_next), which must be the last call
  * on this workspace; ready = 0: they are computed from dyn_in first.
  * payload_next (or NULL at the end of a chain) receives the next step's
- * payload, with the next step's growth factors sched_next.  Results are bit
- * for bit those of mpm_step_fwd.  No reference counterpart (a fusion of
+ * payload, with the next step's growth factors sched_next.  out (MPM_OUT_*)
+ * selects the rows of dyn_out written.  Results are bit for bit those of
+ * mpm_step_fwd in the rows written.  No reference counterpart (a fusion of
  * consecutive transfer.step calls, python/morphompm/integrate.py:15-21). */
 int32_t mpm_step_fwd_chain_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
                                const float* sched, const float* sched_next, float* dyn_out,
                                float* grid, int32_t* sortbuf, float* payload, float* payload_next,
-                               int32_t advect, int32_t ready, void* ws, void* stream);
+                               int32_t advect, int32_t ready, int32_t out, void* ws,
+                               void* stream);
 int32_t mpm_step_fwd_chain_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
                                const double* sched, const double* sched_next, double* dyn_out,
                                double* grid, int32_t* sortbuf, double* payload,
-                               double* payload_next, int32_t advect, int32_t ready, void* ws,
-                               void* stream);
+                               double* payload_next, int32_t advect, int32_t ready, int32_t out,
+                               void* ws, void* stream);
 
 /* Reverse of one step.  Replaces transfer.step_vjp
  * (python/morphompm/transfer.py:82-158).  Inputs are the forward step's

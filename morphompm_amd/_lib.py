@@ -9,12 +9,15 @@ import os
 import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "libmorphompm_hip.so")
+# MORPHOMPM_AMD_LIB names another build of the same library (same ABI), e.g.
+# to compare two builds on one device; the in-tree build is the default.
+LIB_PATH = os.environ.get("MORPHOMPM_AMD_LIB") or os.path.join(HERE, "_lib", "libmorphompm_hip.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 SLOT_ALIGN = 256
 
 NEO_HOOKEAN, HENCKY, HERSCHEL_BULKLEY = 0, 1, 2
+OUT_FULL, OUT_RECORD, OUT_NEXT = 0, 1, 2   # rows of dyn_out a chained step writes
 
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
@@ -74,7 +77,7 @@ def lib():
         f.argtypes = [ctypes.POINTER(MpmBatch), _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P]
         f = getattr(L, f"mpm_step_fwd_chain_{suf}")
         f.restype = _I32
-        f.argtypes = [ctypes.POINTER(MpmBatch)] + [_P] * 9 + [_I32, _I32, _P, _P]
+        f.argtypes = [ctypes.POINTER(MpmBatch)] + [_P] * 9 + [_I32, _I32, _I32, _P, _P]
         f = getattr(L, f"mpm_step_bwd_{suf}")
         f.restype = _I32
         f.argtypes = [ctypes.POINTER(MpmBatch)] + [_P] * 11 + [_I32, _P, _P]

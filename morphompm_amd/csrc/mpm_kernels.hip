@@ -850,7 +850,8 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
                                              T* dyn_out, int advect, const T* Fgp,
                                              const T* sched_next, int TX, int TY, int TZ,
                                              T* payload_next, int32_t* key, int32_t* count,
-                                             int32_t* unit_flag, const int32_t* chunks) {
+                                             int32_t* unit_flag, int out,
+                                             const int32_t* chunks) {
   const int64_t p0 = block_first_slot(chunks), p = p0 + threadIdx.x;
   if (p0 >= n_slots) return;
   const int ii = __builtin_amdgcn_readfirstlane(slot_inst[p0 / kChunk]);
@@ -913,8 +914,8 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
     T xn[3];
     for (int c = 0; c < 3; ++c) xn[c] = advect ? x[c] + I.dt * nv[c] : x[c];
     st3(dyn_out, n_slots, p, RX, xn);
-    st3(dyn_out, n_slots, p, RV, nv);
-    st9(dyn_out, n_slots, p, RC, nC);
+    if (out == MPM_OUT_FULL) st3(dyn_out, n_slots, p, RV, nv);
+    if (out != MPM_OUT_NEXT) st9(dyn_out, n_slots, p, RC, nC);
     st9(dyn_out, n_slots, p, RF, Fn);
     if (NEXT) {
       const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), sched_next ? sched_next[ii] : T(1));
@@ -1516,9 +1517,11 @@ void dispatch_adj(int ti, const mpm_batch* b, const SortView<const int32_t>& sv,
 template <typename T>
 int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sched,
                   const T* sched_next, T* dyn_out, T* grid, int32_t* sortbuf, T* payload,
-                  T* payload_next, int advect, int ready, void* ws, void* stream) {
+                  T* payload_next, int advect, int ready, int out, void* ws, void* stream) {
   int rc = validate(b);
   if (rc) return rc;
+  if (out != MPM_OUT_FULL && out != MPM_OUT_RECORD && out != MPM_OUT_NEXT)
+    return set_err(MPM_EINVAL, "step_fwd: out must be MPM_OUT_FULL, _RECORD or _NEXT (got %d)", out);
   if (!dyn_in || !Fg || !dyn_out || !grid || !sortbuf || !payload || !ws)
     return set_err(MPM_EINVAL, "step_fwd: NULL buffer");
   if (dyn_in == dyn_out) return set_err(MPM_EINVAL, "step_fwd: dyn_out must not alias dyn_in");
@@ -1546,12 +1549,12 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
     per_model(b, bs, [&](auto M, const int32_t* ch, dim3 g) {
       k_g2p<T, decltype(M)::value, true><<<g, bs, 0, st>>>(
           b->inst, b->slot_inst, b->n_slots, dyn_in, grid, dyn_out, advect, Fg, sched_next, TX,
-          TY, TZ, payload_next, w.key, w.count, w.unit_flag, ch);
+          TY, TZ, payload_next, w.key, w.count, w.unit_flag, out, ch);
     });
   else   // the G2P part does not depend on the model
     k_g2p<T, MPM_NEO_HOOKEAN, false><<<grid1(b->n_slots, bs), bs, 0, st>>>(
         b->inst, b->slot_inst, b->n_slots, dyn_in, grid, dyn_out, advect, nullptr, nullptr, TX,
-        TY, TZ, nullptr, nullptr, nullptr, nullptr, nullptr);
+        TY, TZ, nullptr, nullptr, nullptr, nullptr, out, nullptr);
   return check_launch("step_fwd");
 }
 
@@ -1726,28 +1729,29 @@ int32_t mpm_step_fwd_f32(const mpm_batch* b, const float* dyn_in, const float* F
                          const float* sched, float* dyn_out, float* grid, int32_t* sortbuf,
                          float* payload, int32_t advect, void* ws, void* stream) {
   return step_fwd_impl<float>(b, dyn_in, Fg, sched, nullptr, dyn_out, grid, sortbuf, payload,
-                              nullptr, advect, 0, ws, stream);
+                              nullptr, advect, 0, MPM_OUT_FULL, ws, stream);
 }
 int32_t mpm_step_fwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
                          const double* sched, double* dyn_out, double* grid, int32_t* sortbuf,
                          double* payload, int32_t advect, void* ws, void* stream) {
   return step_fwd_impl<double>(b, dyn_in, Fg, sched, nullptr, dyn_out, grid, sortbuf, payload,
-                               nullptr, advect, 0, ws, stream);
+                               nullptr, advect, 0, MPM_OUT_FULL, ws, stream);
 }
 int32_t mpm_step_fwd_chain_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
                                const float* sched, const float* sched_next, float* dyn_out,
                                float* grid, int32_t* sortbuf, float* payload, float* payload_next,
-                               int32_t advect, int32_t ready, void* ws, void* stream) {
+                               int32_t advect, int32_t ready, int32_t out, void* ws,
+                               void* stream) {
   return step_fwd_impl<float>(b, dyn_in, Fg, sched, sched_next, dyn_out, grid, sortbuf, payload,
-                              payload_next, advect, ready, ws, stream);
+                              payload_next, advect, ready, out, ws, stream);
 }
 int32_t mpm_step_fwd_chain_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
                                const double* sched, const double* sched_next, double* dyn_out,
                                double* grid, int32_t* sortbuf, double* payload,
-                               double* payload_next, int32_t advect, int32_t ready, void* ws,
-                               void* stream) {
+                               double* payload_next, int32_t advect, int32_t ready, int32_t out,
+                               void* ws, void* stream) {
   return step_fwd_impl<double>(b, dyn_in, Fg, sched, sched_next, dyn_out, grid, sortbuf, payload,
-                               payload_next, advect, ready, ws, stream);
+                               payload_next, advect, ready, out, ws, stream);
 }
 int32_t mpm_step_bwd_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
                          const float* sched, const float* dyn_out, const float* grid,

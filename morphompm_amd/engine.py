@@ -315,14 +315,16 @@ class BatchPlan:
         check(rc, "step_fwd")
 
     def step_fwd_chain(self, dyn_in, G, sched_row, sched_next, dyn_out, grid, sortbuf, payload,
-                       payload_next, advect, ready, stream=None):
+                       payload_next, advect, ready, out=_lib.OUT_FULL, stream=None):
         """One step of a chain: ``ready`` -- this step's payload and sort keys
         come from the previous chained step; ``payload_next`` (or None)
-        receives the next step's payload."""
+        receives the next step's payload; ``out`` -- the rows of dyn_out
+        written (OUT_FULL, OUT_RECORD: all but v, OUT_NEXT: x and F)."""
         f = getattr(_lib.lib(), f"mpm_step_fwd_chain_{suffix(dyn_in.dtype)}")
         rc = f(ctypes.byref(self.c), ptr(dyn_in), ptr(G), sched_row, sched_next, ptr(dyn_out),
                ptr(grid), ptr(sortbuf), ptr(payload), ptr(payload_next), int(bool(advect)),
-               int(bool(ready)), ptr(self.workspace(dyn_in.dtype)), stream_ptr(stream))
+               int(bool(ready)), int(out), ptr(self.workspace(dyn_in.dtype)),
+               stream_ptr(stream))
         check(rc, "step_fwd_chain")
 
     def step_bwd(self, dyn_in, G, sched_row, dyn_out, grid, sortbuf, payload, bar_out, bar_in,
@@ -424,16 +426,24 @@ class Trajectory:
 
     def _run_segment(self, s, keep):
         """The segment's steps as one chain: each step's G2P writes the next
-        step's payload and sort keys; the first step computes its own."""
+        step's payload and sort keys; the first step computes its own.
+        States written: checkpoints and the final state in full; a kept
+        segment's step records without v (the reverse pass reads x, C, F);
+        the intermediate states of a segment that is not kept with x and F
+        only (the next step reads nothing else)."""
         t0, L = self._seg(s)
         for k in range(L):
             dst = self.seg_dyn[k] if (keep or k < L - 1) else self.ckpt[s + 1]
             last = k == L - 1
+            if last and (not keep or s == self.nseg - 1):
+                out = _lib.OUT_FULL
+            else:
+                out = _lib.OUT_RECORD if keep else _lib.OUT_NEXT
             self.plan.step_fwd_chain(self._src(s, k), self.G, self._a(t0 + k),
                                      None if last else self._a(t0 + k + 1), dst,
                                      self.seg_grid[k], self.seg_sort[k], self.seg_payload[k],
                                      None if last else self.seg_payload[k + 1], self.advect,
-                                     k > 0, self.stream)
+                                     k > 0, out, self.stream)
         self._loaded = s if keep else -1
 
     def forward(self, dyn0, G=None):

@@ -106,9 +106,13 @@ def test_step_fwd_rejects_bad_buffers(suf):
     assert "alias" in _msg()
     g = getattr(L, f"mpm_step_fwd_chain_{suf}")
     b = _batch()
-    rc = g(ctypes.byref(b), FAKE, FAKE + 256, None, None, FAKE + 512, FAKE + 768, FAKE + 1024,
-           FAKE + 1280, FAKE + 1280, 1, 0, FAKE + 4096, None)       # payload_next == payload
-    assert rc == MPM_EINVAL and "payload_next" in _msg()
+    args = [FAKE, FAKE + 256, None, None, FAKE + 512, FAKE + 768, FAKE + 1024, FAKE + 1280,
+            FAKE + 1280, 1, 0, 0, FAKE + 4096, None]                    # payload_next == payload
+    assert g(ctypes.byref(b), *args) == MPM_EINVAL and "payload_next" in _msg()
+    args[8] = FAKE + 1536
+    for out in (-1, 3):                                               # MPM_OUT_* only
+        args[11] = out
+        assert g(ctypes.byref(b), *args) == MPM_EINVAL and "MPM_OUT" in _msg()
 
 
 @pytest.mark.parametrize("suf", ["f32", "f64"])
