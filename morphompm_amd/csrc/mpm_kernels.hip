@@ -298,18 +298,6 @@ __device__ __forceinline__ void count_keys(int32_t k, int32_t* count) {
   if (head && k >= 0) atomicAdd(&count[k], run);
 }
 
-// The same runs on a cursor, returning each lane's place among its bin's
-// particles (k_bin_rank fixes the order in a bin).
-__device__ __forceinline__ int32_t place_in_bin(int32_t k, int32_t* cursor) {
-  bool head;
-  int my_head, run;
-  run_heads(k, head, my_head, run);
-  int32_t base_slot = 0;
-  if (head && k >= 0) base_slot = atomicAdd(&cursor[k], run);
-  base_slot = __shfl(base_slot, my_head);
-  return base_slot + ((int)(threadIdx.x & 63) - my_head);
-}
-
 // Clears the bin counts and unit flags of the whole batch (before keys are
 // computed from scratch: a chained G2P whose counts no sort consumed may have
 // left some behind).
@@ -393,30 +381,53 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_add(int32_t* start, const i
 
 // Bins in arbitrary order within each bin: placed[start[k] + place] = p,
 // with k the compact key unit_rank[unit] * TB + place in tile, written back
-// to key, and the place in the bin from an atomic cursor.  A key outside the
-// occupied units (only possible when a chained step is told its keys are
+// to key, and the place in the bin from an atomic cursor (one atomic per run
+// of equal keys in adjacent lanes).  Each wave takes kPlaceGroups runs of 64
+// slots, so their loads and atomics are in flight together.  A key outside
+// the occupied units (only possible when a chained step is told its keys are
 // ready and they are not) is dropped: wrong, but every index stays in bounds.
-// (Whole waves only leave early: place_in_bin needs every lane.)
-__global__ void k_place(const int32_t* unit_rank, int TB, int64_t n_units, int32_t* key,
-                        const int32_t* start, int32_t* cursor, int32_t* placed,
-                        int64_t n_slots) {
-  const int64_t p0 = (int64_t)blockIdx.x * blockDim.x, p = p0 + threadIdx.x;
-  if (p0 >= n_slots) return;
-  const int32_t raw = key[p];
-  int32_t k = -1;
-  if (raw >= 0) {
-    const int32_t ur = raw / TB < n_units ? unit_rank[raw / TB] : -1;
-    if (ur >= 0) k = ur * TB + raw % TB;
+constexpr int kPlaceGroups = 4;
+__global__ __launch_bounds__(256) void k_place(const int32_t* unit_rank, int TB, int64_t n_units,
+                                               int32_t* key, const int32_t* start,
+                                               int32_t* cursor, int32_t* placed,
+                                               int64_t n_slots) {
+  constexpr int G = kPlaceGroups;
+  const int lane = threadIdx.x & 63;
+  const int64_t w0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 * G);
+  if (w0 >= n_slots) return;  // whole waves (n_slots is a multiple of 256)
+  int32_t raw[G], k[G], s0[G], my_head[G], base[G];
+#pragma unroll
+  for (int i = 0; i < G; ++i) raw[i] = key[w0 + i * 64 + lane];
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    k[i] = -1;
+    if (raw[i] >= 0) {
+      const int32_t ur = raw[i] / TB < n_units ? unit_rank[raw[i] / TB] : -1;
+      if (ur >= 0) k[i] = ur * TB + raw[i] % TB;
+    }
+    s0[i] = k[i] >= 0 ? start[k[i]] : 0;
   }
-  const int32_t place = place_in_bin(k, cursor);
-  if (raw < 0) {
-    placed[p] = (int32_t)p;  // padding slot: the tail of its instance's range, in order
-    return;
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    bool head;
+    int run;
+    run_heads(k[i], head, my_head[i], run);
+    base[i] = 0;
+    if (head && k[i] >= 0) base[i] = atomicAdd(&cursor[k[i]], run);
   }
-  key[p] = k;
-  if (k < 0) return;
-  const int64_t pos = (int64_t)start[k] + place;
-  if (pos >= 0 && pos < n_slots) placed[pos] = (int32_t)p;
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    const int64_t p = w0 + i * 64 + lane;
+    const int32_t place = __shfl(base[i], my_head[i]) + (lane - my_head[i]);
+    if (raw[i] < 0) {
+      placed[p] = (int32_t)p;  // padding slot: the tail of its instance's range, in order
+      continue;
+    }
+    key[p] = k[i];
+    if (k[i] < 0) continue;
+    const int64_t pos = (int64_t)s0[i] + place;
+    if (pos >= 0 && pos < n_slots) placed[pos] = (int32_t)p;
+  }
 }
 
 // Stable order within each bin: the particle at placed position j goes to
@@ -426,24 +437,35 @@ __global__ void k_place(const int32_t* unit_rank, int TB, int64_t n_units, int32
 // than kStableBin particles (no physical cell holds that many; a state that
 // has blown up can put everything in one bin) keeps its placement order.
 constexpr int32_t kStableBin = 2048;
+constexpr int kRankGroups = 4;  // positions per thread, their first loads in flight together
 __global__ __launch_bounds__(256) void k_bin_rank(const int32_t* key, const int32_t* start,
                                                   const int32_t* placed, int32_t* sorted,
                                                   int32_t* rank, int64_t n_slots) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_slots) return;
-  const int32_t v = placed[j];
-  const int32_t k = key[v];
-  int32_t r = (int32_t)j;
-  if (k >= 0) {
-    const int32_t s0 = start[k], s1 = start[k + 1];
-    if (s1 - s0 <= kStableBin) {
-      r = s0;
-#pragma unroll 4
-      for (int32_t i = s0; i < s1; ++i) r += placed[i] < v;
-    }
+  constexpr int G = kRankGroups;
+  const int64_t j0 = (int64_t)blockIdx.x * (256 * G) + threadIdx.x;
+  int32_t v[G], k[G], s0[G], s1[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) v[g] = j0 + g * 256 < n_slots ? placed[j0 + g * 256] : 0;
+#pragma unroll
+  for (int g = 0; g < G; ++g) k[g] = j0 + g * 256 < n_slots ? key[v[g]] : -1;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    s0[g] = k[g] >= 0 ? start[k[g]] : 0;
+    s1[g] = k[g] >= 0 ? start[k[g] + 1] : 0;
   }
-  sorted[r] = v;
-  rank[v] = r;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int64_t j = j0 + g * 256;
+    if (j >= n_slots) continue;
+    int32_t r = (int32_t)j;
+    if (k[g] >= 0 && s1[g] - s0[g] <= kStableBin) {
+      r = s0[g];
+#pragma unroll 4
+      for (int32_t i = s0[g]; i < s1[g]; ++i) r += placed[i] < v[g];
+    }
+    sorted[r] = v[g];
+    rank[v[g]] = r;
+  }
 }
 
 // Record slot 15 carries the particle's cell as bits: (base+2) per axis in 10
@@ -1433,10 +1455,10 @@ int sort_from_keys(const mpm_batch* b, int32_t* sortbuf, const Workspace& w, hip
   k_scan_blocks<<<dim3((unsigned)nb), kScanBlock, 0, st>>>(w.count, v.active_unit, v.start,
                                                             w.cursor, w.bsum, v.n_active, TB);
   k_scan_add<<<dim3((unsigned)nb), kScanBlock, 0, st>>>(v.start, w.bsum, v.n_active, TB);
-  k_place<<<grid1(b->n_slots, bs), bs, 0, st>>>(v.unit_rank, TB, b->n_tiles, w.key, v.start,
-                                                w.cursor, w.placed, b->n_slots);
-  k_bin_rank<<<grid1(b->n_slots, 256), 256, 0, st>>>(w.key, v.start, w.placed, v.sorted, v.rank,
-                                                     b->n_slots);
+  k_place<<<grid1(b->n_slots, 256 * kPlaceGroups), 256, 0, st>>>(
+      v.unit_rank, TB, b->n_tiles, w.key, v.start, w.cursor, w.placed, b->n_slots);
+  k_bin_rank<<<grid1(b->n_slots, 256 * kRankGroups), 256, 0, st>>>(w.key, v.start, w.placed,
+                                                                   v.sorted, v.rank, b->n_slots);
   return check_launch("sort_by_cell");
 }
 
