@@ -30,15 +30,41 @@ HBM_PEAK_GBS = 8000.0
 # Kernel ids of the library's timer (include/morphompm_hip.h) with each
 # kernel's compulsory bytes per launch (FP32): n particles, a active grid
 # nodes, nb bins.  Per particle: records of 4-byte words read + written.
+def g2p_bytes_per_particle(n_steps, K):
+    """Average algorithmic bytes per particle of one G2P launch over an
+    iteration (forward pass + the reverse pass's recompute of all but the last
+    segment), from what each launch reads and writes: x, F, node v in; x, F
+    out, plus C and v where the state is a record or a checkpoint; with a next
+    step in the chain also G in and the next payload (64 B) and key out."""
+    nseg = -(-n_steps // K)
+    seg_len = [min(K, n_steps - s * K) for s in range(nseg)]
+    total, launches = 0, 0
+
+    def launch(out, nxt):
+        rows = {"full": 96, "record": 84, "next": 48}[out]
+        return 48 + rows + (36 + 68 if nxt else 0)
+    for s, L in enumerate(seg_len):                       # forward pass
+        keep = s == nseg - 1
+        for k in range(L):
+            last = k == L - 1
+            out = "full" if last else ("record" if keep else "next")
+            total += launch(out, not last)
+            launches += 1
+    for s, L in enumerate(seg_len[:-1]):                  # recompute of the kept segments
+        for k in range(L):
+            total += launch("record", k < L - 1)
+            launches += 1
+    return total / launches
+
+
 KERNELS = {
-    0: ("sort", lambda n, a, nb: 48 * n + 12 * nb),         # keys in/out, slot, placed, rank,
-                                                             # sorted; counts + starts of bins
+    0: ("sort", lambda n, a, nb: 40 * n + 12 * nb),         # key in/out, start, placed, rank,
+                                                             # sorted; counts, cursors, starts
     1: ("prep_fwd", lambda n, a, nb: 200 * n),              # x v C F G in, payload + key out
                                                              # (first step of each chain only)
     2: ("p2g_fwd_tile", lambda n, a, nb: 68 * n + 16 * a),  # sorted + payload in, partials out
     3: ("grid_fwd", lambda n, a, nb: 32 * a),               # partials in, (m, v) out
-    4: ("g2p", lambda n, a, nb: 248 * n + 16 * a),          # x F G + node v in; x v C F, next
-                                                             # payload + key out
+    4: ("g2p", lambda n, a, nb: G2P_BYTES * n + 16 * a),    # see g2p_bytes_per_particle
     5: ("prep_adj", lambda n, a, nb: 208 * n),              # x F + 4 cotangents in, payload out
     6: ("p2g_adj_tile", lambda n, a, nb: 64 * n + 16 * a),
     7: ("grid_adj", lambda n, a, nb: 48 * a),               # partials + grid in, adjoint grid out
@@ -49,6 +75,7 @@ KERNELS = {
                                                              # the previous adjoint payload out
 }
 ADJ_KERNELS = {5, 6, 7, 8, 9}
+G2P_BYTES = 248.0   # set from the trajectory shape in main()
 
 
 def parse():
@@ -147,6 +174,8 @@ def main():
     for s in scenes:
         s.checkpoint_every = a.checkpoint_every
     n_total = sum(s.n for s in scenes)
+    global G2P_BYTES
+    G2P_BYTES = g2p_bytes_per_particle(a.sim_steps, a.checkpoint_every)
     rates = np.array([s.extra["rate"] for s in scenes])
     print(f"[bench] {len(scenes)} instances, {n_total} particles, {a.sim_steps} steps, "
           f"K={a.checkpoint_every}", file=sys.stderr, flush=True)
