@@ -109,3 +109,43 @@ def test_sort_moves_with_particles():
     order, start_e, ur_e, na_e = expected_sort(plan, dyn2.cpu().numpy())
     assert int(s[-1]) == na_e
     np.testing.assert_array_equal(s[ns:2 * ns], order)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_chained_step_output_rows(dtype):
+    """mpm_step_fwd_chain's `out`: MPM_OUT_RECORD leaves the v rows of dyn_out
+    as they were, MPM_OUT_NEXT leaves v and C; the rows written, the grid, the
+    sort record and the next payload equal the full write bit for bit."""
+    from morphompm_amd import _lib
+    from morphompm_amd import constitutive as K
+    from morphompm_amd.engine import BatchPlan, InstanceSpec, ROW_C, ROW_F, ROW_V, ROW_X
+    from morphompm_amd.scenes import free_swell_cube
+    sc = free_swell_cube(cells=5, ppc=8, seed=2)
+    m = K.NeoHookean(sc.cfg.material)
+    res = {}
+    for out in (_lib.OUT_FULL, _lib.OUT_RECORD, _lib.OUT_NEXT):
+        p = BatchPlan([InstanceSpec.from_cfg(sc.n, sc.cfg, m, sc.pts)])
+        dyn = p.pack_dyn([(sc.pts, 0.01 * np.ones((sc.n, 3)), np.zeros((sc.n, 3, 3)),
+                           np.repeat(np.eye(3)[None], sc.n, 0))], dtype)
+        G = p.pack_mat([np.repeat(1.05 * np.eye(3)[None], sc.n, 0)], dtype)
+        dyn_out = torch.full_like(dyn, 7.0)                 # sentinel in every row
+        grid, sort = p.new_grid(dtype), p.new_sort()
+        pay, pay_next = p.new_payload(dtype), p.new_payload(dtype)
+        p.step_fwd_chain(dyn, G, None, None, dyn_out, grid, sort, pay, pay_next, True, False,
+                         out)
+        torch.cuda.synchronize()
+        res[out] = (dyn_out, grid, sort, pay_next, p.live)
+    full, live = res[_lib.OUT_FULL][0], res[_lib.OUT_FULL][4]
+    assert not torch.any(full[:, live] == 7.0)
+    for out, (d, g, srt, pn, _) in res.items():
+        assert torch.equal(g, res[_lib.OUT_FULL][1]) and torch.equal(srt, res[_lib.OUT_FULL][2])
+        assert torch.equal(pn[live], res[_lib.OUT_FULL][3][live])
+        for r0, r1, written in ((ROW_X, ROW_X + 3, True),
+                                (ROW_V, ROW_V + 3, out == _lib.OUT_FULL),
+                                (ROW_C, ROW_C + 9, out != _lib.OUT_NEXT),
+                                (ROW_F, ROW_F + 9, True)):
+            rows = d[r0:r1][:, live]
+            if written:
+                assert torch.equal(rows, full[r0:r1][:, live]), (out, r0)
+            else:
+                assert torch.all(rows == 7.0), (out, r0)
