@@ -35,29 +35,133 @@ def _soa(t, n, k):
     return t.reshape(n, k).transpose(0, 1)
 
 
-def rollout(x0, v0, C0, F0, Fg, model, cfg, n_steps, advect=False, *, checkpoint_every=None,
-            sched=None, hb_params=None):
-    """Final (x, v, C, F) after n_steps; differentiable.  Inputs are CUDA
-    tensors (n,3)/(n,3,3) of one dtype (float32 or float64)."""
+def _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched):
     n = x0.shape[0]
-    dtype = x0.dtype
     # positions are stored relative to the cell nearest their mean (FP32
-    # keeps more bits of the displacements); the plan is private to the call
+    # keeps more bits of the displacements)
     spec = InstanceSpec.from_cfg(n, cfg, model, x0.detach(), center=True)
     plan = BatchPlan([spec])
-    traj = Trajectory(plan, dtype, n_steps, advect, checkpoint_every, sched)
+    traj = Trajectory(plan, x0.dtype, n_steps, advect, checkpoint_every, sched)
+    return plan, traj
+
+
+def _pack(plan, x0, v0, C0, F0, Fg):
+    n = x0.shape[0]
     sl = plan.sl(0)
     pad = plan.n_slots
-    off = plan.origin_offset(0)
-    xr = (x0.to(torch.float64) - off).to(dtype)
+    xr = (x0.to(torch.float64) - plan.origin_offset(0)).to(x0.dtype)
     dyn0 = torch.cat([_soa(xr, n, 3), _soa(v0, n, 3), _soa(C0, n, 9), _soa(F0, n, 9)], 0)
     dyn0 = torch.nn.functional.pad(dyn0, (sl.start, pad - sl.stop))
     G = torch.nn.functional.pad(_soa(Fg, n, 9), (sl.start, pad - sl.stop))
-    prm_base = plan.specs[0].prm
-    if hb_params is not None and getattr(model, "kind", None) != "hb":
-        raise ValueError("hb_params applies to HerschelBulkley models only")
-    fin = _Rollout.apply(dyn0, G, hb_params, traj, plan, prm_base)
-    f = fin[:, sl]
-    x = (f[ROW_X:ROW_X + 3].T.to(torch.float64) + off).to(dtype)
+    return dyn0, G
+
+
+def _unpack(plan, fin, n, dtype):
+    f = fin[:, plan.sl(0)]
+    x = (f[ROW_X:ROW_X + 3].T.to(torch.float64) + plan.origin_offset(0)).to(dtype)
     return (x, f[ROW_V:ROW_V + 3].T, f[ROW_C:ROW_C + 9].T.reshape(n, 3, 3),
             f[ROW_F:ROW_F + 9].T.reshape(n, 3, 3))
+
+
+def _check_hb(model, hb_params):
+    if hb_params is not None and getattr(model, "kind", None) != "hb":
+        raise ValueError("hb_params applies to HerschelBulkley models only")
+
+
+def rollout(x0, v0, C0, F0, Fg, model, cfg, n_steps, advect=False, *, checkpoint_every=None,
+            sched=None, hb_params=None):
+    """Final (x, v, C, F) after n_steps; differentiable.  Inputs are CUDA
+    tensors (n,3)/(n,3,3) of one dtype (float32 or float64).  The plan and
+    its buffers are private to the call."""
+    _check_hb(model, hb_params)
+    plan, traj = _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched)
+    dyn0, G = _pack(plan, x0, v0, C0, F0, Fg)
+    fin = _Rollout.apply(dyn0, G, hb_params, traj, plan, plan.specs[0].prm)
+    return _unpack(plan, fin, x0.shape[0], x0.dtype)
+
+
+class _GraphRollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, dyn0, G, hb_params, runner):
+        fin = runner._forward(dyn0, G, hb_params)
+        ctx.runner = runner
+        return fin
+
+    @staticmethod
+    def backward(ctx, bar):
+        bar0, G_bar, th = ctx.runner._backward(bar)
+        return bar0, G_bar, th, None
+
+
+class GraphRollout:
+    """Repeated differentiable rollouts of one scene, for inversion loops: the
+    same trajectory as ``rollout`` (bit for bit), but its forward pass and its
+    checkpointed reverse pass are each captured once as a HIP graph and
+    replayed on later calls, which removes the per-step launch work of the
+    eager path.  Built from the rest positions x0 (the layout and tile
+    alignment); each call takes the same arguments as ``rollout`` minus the
+    fixed ones.  Inputs are copied into the runner's own buffers; outputs and
+    gradients are fresh tensors.  One runner serves one call at a time."""
+
+    def __init__(self, x0, model, cfg, n_steps, advect=False, *, checkpoint_every=None,
+                 sched=None, hb=False):
+        if hb and getattr(model, "kind", None) != "hb":
+            raise ValueError("hb applies to HerschelBulkley models only")
+        self.model, self.n, self.dtype, self.hb = model, x0.shape[0], x0.dtype, bool(hb)
+        self.plan, self.traj = _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched)
+        p = self.plan
+        self.prm_base = list(p.specs[0].prm)
+        self.dyn_in = p.new_dyn(self.dtype)
+        self.G_in = p.new_mat(self.dtype)
+        self.bar_in = p.new_dyn(self.dtype)
+        self._g_fwd = self._g_bwd = None
+
+    def _set_theta(self, hb_params):
+        if self.hb:
+            prm = list(self.prm_base)
+            prm[3], prm[1] = float(hb_params[0]), float(hb_params[1])   # tau_y, K
+            self.plan.set_prm(0, prm)
+
+    def _capture(self):
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):          # warm-up outside capture
+            self.traj.forward(self.dyn_in, self.G_in)
+            self.traj.backward(self.bar_in, want_theta=self.hb)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        self._g_fwd = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self._g_fwd):
+            self._fin = self.traj.forward(self.dyn_in, self.G_in)
+        self._seg_after_fwd = self.traj._loaded
+        self._g_bwd = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self._g_bwd):
+            self._out = self.traj.backward(self.bar_in, want_theta=self.hb)
+        self._seg_after_bwd = self.traj._loaded
+
+    def _forward(self, dyn0, G, hb_params):
+        self.dyn_in.copy_(dyn0)
+        self.G_in.copy_(G)
+        self._set_theta(hb_params)
+        if self._g_fwd is None:
+            self._capture()
+        self._g_fwd.replay()
+        self.traj._loaded = self._seg_after_fwd
+        return self._fin.clone()
+
+    def _backward(self, bar):
+        self.bar_in.copy_(bar)
+        self._g_bwd.replay()
+        self.traj._loaded = self._seg_after_bwd
+        bar0, G_bar, theta = self._out
+        th = theta[0].to(bar.dtype) if self.hb else None
+        return bar0.clone(), G_bar.clone(), th
+
+    def __call__(self, x0, v0, C0, F0, Fg, hb_params=None):
+        if self.hb != (hb_params is not None):
+            raise ValueError("hb_params must be given exactly when the runner was built with hb=True")
+        if x0.shape[0] != self.n or x0.dtype != self.dtype:
+            raise ValueError(f"runner built for {self.n} particles of {self.dtype}")
+        dyn0, G = _pack(self.plan, x0, v0, C0, F0, Fg)
+        fin = _GraphRollout.apply(dyn0, G, hb_params, self)
+        return _unpack(self.plan, fin, self.n, self.dtype)

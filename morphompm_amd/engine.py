@@ -172,6 +172,10 @@ class BatchPlan:
                 c.model_chunk_off[m] = int(off[m])
         self.c = c
         self.sortlen = 2 * self.n_slots + self.n_bins + 1 + 2 * self.n_tiles + 1
+        # each instance's chunks are contiguous: [start, end) in chunk units
+        cnt = np.bincount(np.array(slot_inst, dtype=np.int64), minlength=self.n_inst)
+        self._chunk_end = torch.from_numpy(np.cumsum(cnt)).to(self.device)
+        self._chunk_start = torch.from_numpy(np.cumsum(cnt) - cnt).to(self.device)
         self._ws = {}
         # slot -> live mask and per-chunk instance (host copies for packing)
         self.slot_inst_np = np.repeat(np.array(slot_inst, dtype=np.int64), SLOT_ALIGN)
@@ -354,15 +358,11 @@ class BatchPlan:
                 stream_ptr(stream)), "sort_by_cell")
 
     def theta_per_instance(self, theta_acc):
-        """theta_acc [n_slots/256, 2] float64 -> (n_inst, 2) in chunk order."""
-        chunk_inst = torch.from_numpy(self.slot_inst_np[::SLOT_ALIGN].copy()).to(self.device)
-        out = torch.zeros(self.n_inst, 2, dtype=torch.float64, device=self.device)
+        """theta_acc [n_slots/256, 2] float64 -> (n_inst, 2), summed over each
+        instance's chunks in chunk order (device work only: safe under graph
+        capture)."""
         cs = torch.cumsum(torch.cat([theta_acc.new_zeros(1, 2), theta_acc]), 0)
-        counts = torch.bincount(chunk_inst, minlength=self.n_inst)
-        ends = torch.cumsum(counts, 0)
-        starts = ends - counts
-        out[:] = cs[ends] - cs[starts]
-        return out
+        return cs[self._chunk_end] - cs[self._chunk_start]
 
 
 def _dev(a, dtype, device):
