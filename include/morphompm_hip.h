@@ -205,9 +205,12 @@ int32_t mpm_step_bwd_f64(const mpm_batch* b, const double* dyn_in, const double*
  * step t+1), which must be the last call on this workspace; ready = 0: it is
  * computed from dyn_in and bar_out first.  dyn_prev / sortbuf_prev (both or
  * NULL): step t-1's input state and sort record; the constitutive-VJP kernel
- * then also writes step t-1's payload into ws.  Results are bit for bit
- * those of mpm_step_bwd.  No reference counterpart (a fusion of consecutive
- * transfer.step_vjp calls, python/morphompm/integrate.py:24-28). */
+ * then also writes step t-1's payload into ws, which carries C_bar, and the C
+ * rows of bar_in are not written (the chain reads them from nowhere else; a
+ * caller that needs C_bar of an intermediate step passes NULL).  Results are
+ * bit for bit those of mpm_step_bwd in the rows written.  No reference
+ * counterpart (a fusion of consecutive transfer.step_vjp calls,
+ * python/morphompm/integrate.py:24-28). */
 int32_t mpm_step_bwd_chain_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
                                const float* sched, const float* dyn_out, const float* grid,
                                const int32_t* sortbuf, const float* payload, const float* bar_out,

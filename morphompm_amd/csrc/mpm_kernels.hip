@@ -1196,7 +1196,9 @@ __global__ __launch_bounds__(256) void k_adj_vjp(const mpm_instance* inst, const
     }
     M3<T> Cbo = scaled(MD, I.pmass);
     for (int k = 0; k < 9; ++k) Cbo.a[k] += Cbar.a[k];
-    st9(bar_in, n_slots, p, RC, Cbo);
+    // with PREV, C_bar goes into the previous step's payload below and no
+    // later kernel of the chain reads it from bar_in
+    if (!PREV) st9(bar_in, n_slots, p, RC, Cbo);
     {
       const M3<T> Cn = ld9(dyn_out, n_slots, p, RC);
       const M3<T> Fnb = ld9(bar_out, n_slots, p, RF);
