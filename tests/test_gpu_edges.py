@@ -205,3 +205,23 @@ def test_batch_with_empty_and_single_instances_bitwise_fp32():
         b = [t.cpu() for t in bi.plan.unpack_dyn(bi.final_state(), i, bi.perm)]
         for u, w in zip(a, b):
             assert torch.equal(u, w), i
+
+
+@pytest.mark.parametrize("adv", [0, 1])
+def test_grid_record_matches_oracle_grid(adv):
+    """The step's grid record (m, v per node) against the oracle's gm and gv at
+    every node any particle's stencil reaches (the oracle's gm > 0), for the
+    edge scene."""
+    from morphompm_amd import transfer
+    from morphompm_amd.state import ParticleState
+    gcfg, rcfg = _cfgs()
+    arrs = _edge_state()
+    _, cache = transfer.step(ParticleState(*(a.copy() for a in arrs)), _models()[0]["hencky"],
+                             gcfg, advect=bool(adv))
+    _, rc = RV.step(RL.ParticleState(*(a.copy() for a in arrs)), _models()[1]["hencky"], rcfg,
+                    advect=bool(adv))
+    g = cache.grid[:int(np.prod(gcfg.shape))].cpu().numpy()
+    reach = rc["gm"] > 0
+    assert reach.sum() > 100
+    assert _rel(g[reach, 0], rc["gm"][reach]) < 1e-12
+    assert _rel(g[reach, 1:], rc["gv"][reach]) < 1e-12
