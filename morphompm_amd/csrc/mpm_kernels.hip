@@ -1278,7 +1278,7 @@ __global__ __launch_bounds__(256) void k_stress_vjp(ModelParams<T> mp, int64_t n
 struct TileShape {
   int x, y, z;
 };
-constexpr TileShape kTiles[] = {{4, 8, 8}, {4, 4, 16}, {8, 8, 4}, {4, 4, 8}};
+constexpr TileShape kTiles[] = {{4, 8, 8}, {4, 4, 16}, {8, 8, 4}, {4, 4, 8}, {4, 4, 4}, {2, 4, 8}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 int tile_index(const mpm_batch* b) {
@@ -1518,6 +1518,8 @@ void dispatch_fwd(int ti, const mpm_batch* b, const SortView<const int32_t>& sv,
     case 0: launch_tile_fwd<T, 4, 8, 8>(b, sv, w, payload, grid, st); break;
     case 1: launch_tile_fwd<T, 4, 4, 16>(b, sv, w, payload, grid, st); break;
     case 2: launch_tile_fwd<T, 8, 8, 4>(b, sv, w, payload, grid, st); break;
+    case 4: launch_tile_fwd<T, 4, 4, 4>(b, sv, w, payload, grid, st); break;
+    case 5: launch_tile_fwd<T, 2, 4, 8>(b, sv, w, payload, grid, st); break;
     default: launch_tile_fwd<T, 4, 4, 8>(b, sv, w, payload, grid, st); break;
   }
 }
@@ -1529,6 +1531,8 @@ void dispatch_adj(int ti, const mpm_batch* b, const SortView<const int32_t>& sv,
     case 0: launch_tile_adj<T, 4, 8, 8>(b, sv, w, grid, advect, st); break;
     case 1: launch_tile_adj<T, 4, 4, 16>(b, sv, w, grid, advect, st); break;
     case 2: launch_tile_adj<T, 8, 8, 4>(b, sv, w, grid, advect, st); break;
+    case 4: launch_tile_adj<T, 4, 4, 4>(b, sv, w, grid, advect, st); break;
+    case 5: launch_tile_adj<T, 2, 4, 8>(b, sv, w, grid, advect, st); break;
     default: launch_tile_adj<T, 4, 4, 8>(b, sv, w, grid, advect, st); break;
   }
 }

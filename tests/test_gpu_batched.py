@@ -75,6 +75,22 @@ def test_batched_fp64_matches_oracle(models):
         assert abs(g[i] - go) <= 1e-9 * abs(go), (i, g[i], go)
 
 
+def test_every_tile_shape_matches_oracle_fp64():
+    """Each built P2G tile shape (the autotuner's choices) gives the oracle's
+    losses and gradients in FP64."""
+    from morphompm_amd import _lib
+    from morphompm_amd.batched import BatchedInverse
+    sc = _small_batch(("nh", "hencky"), n_steps=8)
+    ref = [_oracle_instance(s, s.extra["rate"]) for s in sc]
+    for tile in _lib.tile_shapes():
+        bi = BatchedInverse(sc, dtype=torch.float64, tile=tuple(tile), use_graphs=False)
+        bi.set_theta([s.extra["rate"] for s in sc])
+        L, g = (t.cpu().numpy() for t in bi.loss_and_grad())
+        for i, (_, Lo, go) in enumerate(ref):
+            assert abs(L[i] - Lo) <= 1e-12 * abs(Lo), (tile, i)
+            assert abs(g[i] - go) <= 1e-9 * abs(go), (tile, i, g[i], go)
+
+
 def test_batched_fp32_error_vs_fp64():
     sc = _small_batch(("nh", "hencky", "nh", "hencky"), n_steps=30)
     _, L32, g32 = _run_batch(sc, torch.float32)

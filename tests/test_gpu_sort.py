@@ -69,7 +69,7 @@ def _plan_and_dyn(dtype, tile=(4, 8, 8), seed=0):
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-@pytest.mark.parametrize("tile", [(4, 8, 8), (4, 4, 8)])
+@pytest.mark.parametrize("tile", [(4, 8, 8), (4, 4, 8), (4, 4, 4), (2, 4, 8)])
 def test_sort_by_cell_matches_contract(dtype, tile):
     plan, dyn = _plan_and_dyn(dtype, tile)
     sb = plan.new_sort()
