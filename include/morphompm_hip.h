@@ -229,9 +229,10 @@ int32_t mpm_step_bwd_chain_f64(const mpm_batch* b, const double* dyn_in, const d
  * `events` holds 2*n_pairs hipEvent_t handles created with timing enabled.
  * Every later launch of kernel `kernel_id` records events[2i] just before and
  * events[2i+1] just after it on its stream, for the i-th such launch while
- * i < n_pairs; inside stream capture they become external event-record nodes,
- * so each graph replay re-records them.  kernel_id < 0 disarms.  Process-wide
- * state, not thread-safe.  mpm_kernel_timer_count() = launches recorded. */
+ * i < n_pairs; launches under stream capture are not timed (event records
+ * are refused inside a capture on this stack).  kernel_id < 0 disarms.
+ * Process-wide state, not thread-safe.  mpm_kernel_timer_count() = launches
+ * recorded. */
 enum {
   MPM_K_SORT = 0,        /* the whole counting-sort sequence */
   MPM_K_PREP_FWD = 1,

@@ -1410,12 +1410,11 @@ struct KernelTimer {
 };
 KernelTimer g_timer;
 
-void record_event(hipEvent_t e, hipStream_t st) {
+// Launches under stream capture are not timed: this stack refuses event
+// records (external or not) inside a capture.
+bool capturing(hipStream_t st) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive)
-    (void)hipEventRecordWithFlags(e, st, hipEventRecordExternal);
-  else
-    (void)hipEventRecord(e, st);
+  return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
 }
 
 struct TimedLaunch {
@@ -1423,15 +1422,15 @@ struct TimedLaunch {
   hipStream_t st;
   hipEvent_t stop;
   TimedLaunch(int id, hipStream_t s) : on(false), st(s), stop(nullptr) {
-    if (g_timer.id == id && g_timer.count < g_timer.n_pairs) {
+    if (g_timer.id == id && g_timer.count < g_timer.n_pairs && !capturing(st)) {
       on = true;
-      record_event(g_timer.ev[2 * g_timer.count], st);
+      (void)hipEventRecord(g_timer.ev[2 * g_timer.count], st);
       stop = g_timer.ev[2 * g_timer.count + 1];
       ++g_timer.count;
     }
   }
   ~TimedLaunch() {
-    if (on) record_event(stop, st);
+    if (on) (void)hipEventRecord(stop, st);
   }
 };
 

@@ -211,3 +211,37 @@ def test_batched_infer_recovers_rates():
         if app:
             seq.append(g)
     np.testing.assert_allclose(hist[k], seq, rtol=0, atol=1e-12)
+
+
+def test_kernel_timer_eager_and_under_capture():
+    """mpm_kernel_timer: eager launches of the armed kernel are timed; an
+    armed timer does not disturb stream capture (captured launches are not
+    timed) and the replays match the eager result."""
+    import ctypes
+    from morphompm_amd import _lib
+    from morphompm_amd.batched import BatchedInverse
+    L = _lib.lib()
+    sc = _small_batch(("nh", "hencky"), n_steps=6)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(64)]
+    for e in evs:
+        e.record()
+    torch.cuda.synchronize()
+    h = (ctypes.c_void_p * 64)(*[e.cuda_event for e in evs])
+    eager = BatchedInverse(sc, dtype=torch.float32, use_graphs=False)
+    eager.set_theta([s.extra["rate"] for s in sc])
+    assert L.mpm_kernel_timer(4, h, 32) == 0          # G2P
+    try:
+        Le, ge = eager.loss_and_grad()
+        torch.cuda.synchronize()
+        n_eager = L.mpm_kernel_timer_count()
+        assert n_eager > 0
+        assert all(evs[2 * i].elapsed_time(evs[2 * i + 1]) > 0 for i in range(n_eager))
+        graph = BatchedInverse(sc, dtype=torch.float32, use_graphs=True)
+        graph.set_theta([s.extra["rate"] for s in sc])
+        L.mpm_kernel_timer(4, h, 32)
+        Lg, gg = graph.loss_and_grad()                 # warm-up (eager, timed), capture, replay
+        torch.cuda.synchronize()
+        assert torch.equal(Le, Lg) and torch.equal(ge, gg)
+        assert L.mpm_kernel_timer_count() == n_eager   # only the eager warm-up was timed
+    finally:
+        L.mpm_kernel_timer(-1, None, 0)
