@@ -57,6 +57,18 @@ def g2p_bytes_per_particle(n_steps, K):
     return total / launches
 
 
+def adj_vjp_bytes_per_particle(n_steps, K):
+    """Average algorithmic bytes per particle of one constitutive-VJP launch
+    over an iteration: MD, C, F, G, C_new, Fn_b and G_b in, F_b and G_b out
+    (256 + 108 B); a launch that also prepares the previous step's adjoint
+    payload reads x, F of that step, x_b, v_b and its rank (76 B) and writes
+    the 64 B payload instead of the C_b rows (36 B): 468 B."""
+    nseg = -(-n_steps // K)
+    seg_len = [min(K, n_steps - s * K) for s in range(nseg)]
+    total = sum((L - 1) * 468 + 364 for L in seg_len)
+    return total / n_steps
+
+
 KERNELS = {
     0: ("sort", lambda n, a, nb: 40 * n + 12 * nb),         # key in/out, start, placed, rank,
                                                              # sorted; counts, cursors, starts
@@ -70,12 +82,11 @@ KERNELS = {
     7: ("grid_adj", lambda n, a, nb: 48 * a),               # partials + grid in, adjoint grid out
     8: ("adj_gather", lambda n, a, nb: 204 * n + 32 * a),   # sorted, 2 payloads, xn_b, nodes in;
                                                              # x_b v_b MD out
-    9: ("adj_vjp", lambda n, a, nb: 504 * n),               # MD C F G Cn Fn_b G_b, previous
-                                                             # x F, x_b v_b in; C_b F_b G_b and
-                                                             # the previous adjoint payload out
+    9: ("adj_vjp", lambda n, a, nb: ADJ_VJP_BYTES * n),     # see adj_vjp_bytes_per_particle
 }
 ADJ_KERNELS = {5, 6, 7, 8, 9}
 G2P_BYTES = 248.0   # set from the trajectory shape in main()
+ADJ_VJP_BYTES = 468.0
 
 
 def parse():
@@ -174,8 +185,9 @@ def main():
     for s in scenes:
         s.checkpoint_every = a.checkpoint_every
     n_total = sum(s.n for s in scenes)
-    global G2P_BYTES
+    global G2P_BYTES, ADJ_VJP_BYTES
     G2P_BYTES = g2p_bytes_per_particle(a.sim_steps, a.checkpoint_every)
+    ADJ_VJP_BYTES = adj_vjp_bytes_per_particle(a.sim_steps, a.checkpoint_every)
     rates = np.array([s.extra["rate"] for s in scenes])
     print(f"[bench] {len(scenes)} instances, {n_total} particles, {a.sim_steps} steps, "
           f"K={a.checkpoint_every}", file=sys.stderr, flush=True)
