@@ -114,8 +114,9 @@ const char* mpm_last_error(void);
 /* Bytes of scratch `ws` the step functions need for this batch and dtype
  * (0 = f32, 1 = f64).  Negative on error.  `ws` must be zero-filled once
  * before its first use; every sort leaves its bin counters and unit flags
- * cleared again, so a workspace is reused across steps (one stream at a time)
- * without further clearing. */
+ * cleared again, and a step that computes its sort keys from scratch clears
+ * any a chained G2P left, so a workspace is reused across steps (one stream
+ * at a time) without further clearing. */
 int64_t mpm_workspace_bytes(const mpm_batch* b, int32_t dtype);
 
 /* Tile shapes this build instantiates, as x,y,z triples; returns the count. */
