@@ -1026,8 +1026,8 @@ __global__ __launch_bounds__(256) void k_grid_adj(const mpm_instance* inst, cons
 //   nv_b + 4 inv_dx^2 C_b' dpos = q1 + P1 o   (P1 = 4 inv_dx^2 dx C_b')
 //   pmass v + A dpos            = q2 + A' o   (A' = A dx, A = pmass C + dt c tau)
 // so x_bar's grad-w coefficient is v_i.(q1 + P1 o) + mb_i.(q2 + A' o) + gm_b,i pmass.
-// Writes x_bar, v_bar of the particle and MD = sum w mb dpos^T (row-major, in
-// slot order) for the VJP kernel.
+// Writes x_bar, v_bar of the particle and MD = sum w mb dpos^T (row-major, at
+// the sorted position j; the VJP kernel reads it through rank[p]).
 template <typename T>
 __global__ __launch_bounds__(256) void k_adj_gather(const mpm_instance* inst,
                                                     const int32_t* slot_inst, int64_t n_slots,
