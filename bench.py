@@ -101,6 +101,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--no-keep-all", action="store_true",
+                    help="skip the reference measurement with every segment's step records "
+                         "kept in HBM (no recompute)")
     ap.add_argument("--tile", default="auto", help='P2G tile "x,y,z" or "auto" (autotuner)')
     ap.add_argument("--block", default="auto", help='particle-kernel block size or "auto"')
     ap.add_argument("--pmc-pass", action="store_true",
@@ -315,6 +318,35 @@ def main():
              "achieved_GBs": b_alg * value / 1e9,
              "frac": b_alg * value / 1e9 / HBM_PEAK_GBS}
 
+    # For reference (not the metric): the same iteration with the forward
+    # pass keeping every segment's step records in HBM, so the reverse pass
+    # recomputes nothing (checkpoints are still written every K steps).
+    keep_all = None
+    if not a.no_keep_all:
+        nseg = -(-a.sim_steps // a.checkpoint_every)
+        bk = BatchedInverse(scenes, dtype=torch.float32, tile=tile, block=block,
+                            use_graphs=True, keep_segments=nseg)
+        st = bk.traj._stores
+        hbm = sum(t.numel() * t.element_size() for store in st for t in store)
+        bk.set_theta(theta)
+        bk.loss_and_grad()                         # capture + warm-up
+        torch.cuda.synchronize()
+        k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        k0.record()
+        for _ in range(a.steps):
+            bk.set_theta(theta)
+            bk.loss_and_grad()
+        k1.record()
+        torch.cuda.synchronize()
+        ms_k = k0.elapsed_time(k1) / a.steps
+        keep_all = {"value": ps_per_iter / (ms_k / 1e3), "ms_per_step": ms_k,
+                    "keep_segments": nseg, "segment_stores_GB": hbm / 1e9,
+                    "note": "reference only: no checkpoint recompute in the reverse pass"}
+        del bk, st
+        torch.cuda.empty_cache()
+        print(f"[bench] every segment kept: {ms_k:.2f} ms per iteration "
+              f"({hbm / 1e9:.0f} GB of step records)", file=sys.stderr, flush=True)
+
     # plain single-instance path: same kernels, one instance at a time, no graph
     plain = None
     if a.plain_instances > 0:
@@ -383,6 +415,7 @@ def main():
            "roofline": roof, "whole_step_roofline": whole, "kernels": ktab,
            "cpu_baseline": cpu,
            "plain_gpu": plain,
+           "no_recompute": keep_all,
            "batched_over_plain": (value / plain["per_instance_value"]) if plain else None,
            "batched_over_cpu": (value / cpu["value"]) if cpu else None,
            "fp32_vs_fp64": acc}

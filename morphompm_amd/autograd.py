@@ -35,13 +35,14 @@ def _soa(t, n, k):
     return t.reshape(n, k).transpose(0, 1)
 
 
-def _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched):
+def _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched, keep_segments=1):
     n = x0.shape[0]
     # positions are stored relative to the cell nearest their mean (FP32
     # keeps more bits of the displacements)
     spec = InstanceSpec.from_cfg(n, cfg, model, x0.detach(), center=True)
     plan = BatchPlan([spec])
-    traj = Trajectory(plan, x0.dtype, n_steps, advect, checkpoint_every, sched)
+    traj = Trajectory(plan, x0.dtype, n_steps, advect, checkpoint_every, sched,
+                      keep_segments=keep_segments)
     return plan, traj
 
 
@@ -101,14 +102,17 @@ class GraphRollout:
     eager path.  Built from the rest positions x0 (the layout and tile
     alignment); each call takes the same arguments as ``rollout`` minus the
     fixed ones.  Inputs are copied into the runner's own buffers; outputs and
-    gradients are fresh tensors.  One runner serves one call at a time."""
+    gradients are fresh tensors.  One runner serves one call at a time.
+    ``keep_segments``: step-record stores kept from the forward pass (see
+    ``engine.Trajectory``); more trade HBM for less recompute."""
 
     def __init__(self, x0, model, cfg, n_steps, advect=False, *, checkpoint_every=None,
-                 sched=None, hb=False):
+                 sched=None, hb=False, keep_segments=1):
         if hb and getattr(model, "kind", None) != "hb":
             raise ValueError("hb applies to HerschelBulkley models only")
         self.model, self.n, self.dtype, self.hb = model, x0.shape[0], x0.dtype, bool(hb)
-        self.plan, self.traj = _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched)
+        self.plan, self.traj = _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched,
+                                      keep_segments)
         p = self.plan
         self.prm_base = list(p.specs[0].prm)
         self.dyn_in = p.new_dyn(self.dtype)
@@ -133,11 +137,11 @@ class GraphRollout:
         self._g_fwd = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self._g_fwd):
             self._fin = self.traj.forward(self.dyn_in, self.G_in)
-        self._seg_after_fwd = self.traj._loaded
+        self._seg_after_fwd = self.traj.store_state()
         self._g_bwd = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self._g_bwd):
             self._out = self.traj.backward(self.bar_in, want_theta=self.hb)
-        self._seg_after_bwd = self.traj._loaded
+        self._seg_after_bwd = self.traj.store_state()
 
     def _forward(self, dyn0, G, hb_params):
         self.dyn_in.copy_(dyn0)
@@ -146,13 +150,13 @@ class GraphRollout:
         if self._g_fwd is None:
             self._capture()
         self._g_fwd.replay()
-        self.traj._loaded = self._seg_after_fwd
+        self.traj.set_store_state(self._seg_after_fwd)
         return self._fin.clone()
 
     def _backward(self, bar):
         self.bar_in.copy_(bar)
         self._g_bwd.replay()
-        self.traj._loaded = self._seg_after_bwd
+        self.traj.set_store_state(self._seg_after_bwd)
         bar0, G_bar, theta = self._out
         th = theta[0].to(bar.dtype) if self.hb else None
         return bar0.clone(), G_bar.clone(), th

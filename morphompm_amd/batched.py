@@ -39,10 +39,11 @@ def cofactor_rows(F):
 
 class BatchedInverse:
     def __init__(self, scenes, dtype=torch.float32, tile=DEFAULT_TILE, block=256,
-                 use_graphs=True, checkpoint_every=None):
+                 use_graphs=True, checkpoint_every=None, keep_segments=1):
         """``tile`` / ``block``: the P2G tile shape and particle-kernel block
         size, or "auto" to pick them with the autotuner (cached per shape
-        class)."""
+        class).  ``keep_segments``: step-record stores the forward pass keeps
+        (see ``engine.Trajectory``); more trade HBM for less recompute."""
         self.scenes = scenes
         steps = {s.n_steps for s in scenes}
         if len(steps) != 1:
@@ -67,7 +68,8 @@ class BatchedInverse:
         sched = np.stack([s.sched if s.sched is not None else np.ones(self.n_steps)
                           for s in scenes], 1)
         self.traj = Trajectory(p, dtype, self.n_steps, self.advect, K,
-                               None if np.all(sched == 1.0) else sched)
+                               None if np.all(sched == 1.0) else sched,
+                               keep_segments=keep_segments)
         n = [s.n for s in scenes]
         self.perm = p.cell_order([s.pts for s in scenes])
         self.dyn0 = p.pack_dyn([(s.pts, np.zeros((k, 3)), np.zeros((k, 3, 3)),
@@ -116,11 +118,11 @@ class BatchedInverse:
         self._g_fwd = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self._g_fwd):
             self._fwd()
-        self._seg_after_fwd = self.traj._loaded
+        self._seg_after_fwd = self.traj.store_state()
         self._g_bwd = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self._g_bwd):
             self._bwd()
-        self._seg_after_bwd = self.traj._loaded
+        self._seg_after_bwd = self.traj.store_state()
 
     def run_forward(self):
         if self.use_graphs:
@@ -128,7 +130,7 @@ class BatchedInverse:
                 self._capture()
             self._g_fwd.replay()
             # a replay leaves the segment store as the captured run did
-            self.traj._loaded = self._seg_after_fwd
+            self.traj.set_store_state(self._seg_after_fwd)
         else:
             self._fwd()
         return self._final
@@ -138,7 +140,7 @@ class BatchedInverse:
             if self._g_bwd is None:
                 self._capture()
             self._g_bwd.replay()
-            self.traj._loaded = self._seg_after_bwd
+            self.traj.set_store_state(self._seg_after_bwd)
         else:
             self._bwd()
         return self.traj.G_bar

@@ -376,10 +376,17 @@ class Trajectory:
 
     ``sched``: None (growth tensor used as is, the reference semantics) or a
     (n_steps, n_inst) tensor a_t: step t uses Fg = I + a_t (G - I).
+
+    ``keep_segments`` (M, default 1): the forward pass keeps the step records
+    of its last M segments, and the reverse pass recomputes only the others
+    (M >= the number of segments: no recompute).  Each kept segment costs one
+    segment store (K steps of state, grid, sort record and payload: about
+    10 GB for 2.7 M particles at K = 20); checkpoints are written every K
+    steps either way.
     """
 
     def __init__(self, plan, dtype, n_steps, advect, checkpoint_every=None, sched=None,
-                 stream=None):
+                 stream=None, keep_segments=1):
         self.plan = plan
         self.dtype = dtype
         self.n_steps = int(n_steps)
@@ -399,17 +406,28 @@ class Trajectory:
         p = plan
         self.ckpt = torch.zeros(max(1, self.nseg), N_DYN, p.n_slots, dtype=dtype, device=p.device)
         Ks = self.K
-        self.seg_dyn = torch.zeros(Ks, N_DYN, p.n_slots, dtype=dtype, device=p.device)
-        self.seg_grid = torch.zeros(Ks, p.n_nodes, 4, dtype=dtype, device=p.device)
-        self.seg_sort = torch.zeros(Ks, p.sortlen, dtype=torch.int32, device=p.device)
-        self.seg_payload = torch.zeros(Ks, p.n_slots, 16, dtype=dtype, device=p.device)
+        self.M = max(1, min(int(keep_segments), max(1, self.nseg)))
+        self._stores = [(torch.zeros(Ks, N_DYN, p.n_slots, dtype=dtype, device=p.device),
+                         torch.zeros(Ks, p.n_nodes, 4, dtype=dtype, device=p.device),
+                         torch.zeros(Ks, p.sortlen, dtype=torch.int32, device=p.device),
+                         torch.zeros(Ks, p.n_slots, 16, dtype=dtype, device=p.device))
+                        for _ in range(self.M)]
+        # the store that holds the last segment
+        self.seg_dyn, self.seg_grid, self.seg_sort, self.seg_payload = self._stores[-1]
         self.G = p.new_mat(dtype)
         self.bar_a = p.new_dyn(dtype)
         self.bar_b = p.new_dyn(dtype)
         self.G_bar = p.new_mat(dtype)
         self.theta_acc = torch.zeros(p.n_slots // SLOT_ALIGN, 2, dtype=torch.float64,
                                      device=p.device)
-        self._loaded = -1      # segment whose steps sit in seg_*
+        self._held = [-1] * self.M     # segment whose step records each store holds
+
+    # segment-store bookkeeping (graph replays restore it as the captured run left it)
+    def store_state(self):
+        return tuple(self._held)
+
+    def set_store_state(self, state):
+        self._held = list(state)
 
     def _a(self, t):
         if self.sched is None:
@@ -421,8 +439,22 @@ class Trajectory:
         t0 = s * self.K
         return t0, min(self.K, self.n_steps - t0)
 
+    def _store(self, s):
+        """Store index of segment s: the last M segments have one each; the
+        others are (re)computed in store 0, which is free by then."""
+        return max(0, s - (self.nseg - self.M))
+
+    def _dst(self, s, k):
+        """The state after step k of segment s: the next checkpoint after the
+        segment's last step (the final state stays in its store), else the
+        segment's store."""
+        _, L = self._seg(s)
+        if k == L - 1 and s < self.nseg - 1:
+            return self.ckpt[s + 1]
+        return self._stores[self._store(s)][0][k]
+
     def _src(self, s, k):
-        return self.ckpt[s] if k == 0 else self.seg_dyn[k - 1]
+        return self.ckpt[s] if k == 0 else self._dst(s, k - 1)
 
     def _run_segment(self, s, keep):
         """The segment's steps as one chain: each step's G2P writes the next
@@ -430,21 +462,23 @@ class Trajectory:
         States written: checkpoints and the final state in full; a kept
         segment's step records without v (the reverse pass reads x, C, F);
         the intermediate states of a segment that is not kept with x and F
-        only (the next step reads nothing else)."""
+        only (the next step reads nothing else).  A recomputed segment
+        rewrites its closing checkpoint with the same bits."""
         t0, L = self._seg(s)
+        m = self._store(s)
+        _, grid, sort, payload = self._stores[m]
         for k in range(L):
-            dst = self.seg_dyn[k] if (keep or k < L - 1) else self.ckpt[s + 1]
             last = k == L - 1
-            if last and (not keep or s == self.nseg - 1):
+            if last:
                 out = _lib.OUT_FULL
             else:
                 out = _lib.OUT_RECORD if keep else _lib.OUT_NEXT
             self.plan.step_fwd_chain(self._src(s, k), self.G, self._a(t0 + k),
-                                     None if last else self._a(t0 + k + 1), dst,
-                                     self.seg_grid[k], self.seg_sort[k], self.seg_payload[k],
-                                     None if last else self.seg_payload[k + 1], self.advect,
+                                     None if last else self._a(t0 + k + 1), self._dst(s, k),
+                                     grid[k], sort[k], payload[k],
+                                     None if last else payload[k + 1], self.advect,
                                      k > 0, out, self.stream)
-        self._loaded = s if keep else -1
+        self._held[m] = s if keep else -1
 
     def forward(self, dyn0, G=None):
         """Runs the trajectory from dyn0 ([24, n_slots]); returns the final state
@@ -455,16 +489,17 @@ class Trajectory:
         if self.n_steps == 0:
             return self.ckpt[0]
         for s in range(self.nseg):
-            self._run_segment(s, keep=(s == self.nseg - 1))
+            self._run_segment(s, keep=(s >= self.nseg - self.M))
         return self.final()
 
     def final(self):
         if self.n_steps == 0:
             return self.ckpt[0]
-        _, L = self._seg(self.nseg - 1)
-        if self._loaded != self.nseg - 1:
-            self._run_segment(self.nseg - 1, keep=True)
-        return self.seg_dyn[L - 1]
+        s = self.nseg - 1
+        _, L = self._seg(s)
+        if self._held[self._store(s)] != s:
+            self._run_segment(s, keep=True)
+        return self._dst(s, L - 1)
 
     def backward(self, bar_final, G_bar_final=None, want_theta=False):
         """Cotangents of (dyn0, G) from the cotangent of the final state.
@@ -484,18 +519,19 @@ class Trajectory:
         bar_out, bar_in = self.bar_a, self.bar_b
         th = self.theta_acc if want_theta else None
         for s in range(self.nseg - 1, -1, -1):
-            if self._loaded != s:
+            m = self._store(s)
+            if self._held[m] != s:
                 self._run_segment(s, keep=True)
             t0, L = self._seg(s)
+            _, grid, sort, payload = self._stores[m]
             # one chain per segment: each reverse step's VJP kernel prepares
             # the adjoint payload of the step before it
             for k in range(L - 1, -1, -1):
                 self.plan.step_bwd_chain(self._src(s, k), self.G, self._a(t0 + k),
-                                         self.seg_dyn[k], self.seg_grid[k], self.seg_sort[k],
-                                         self.seg_payload[k], bar_out, bar_in, self.G_bar, th,
-                                         self.advect, k < L - 1,
+                                         self._dst(s, k), grid[k], sort[k], payload[k],
+                                         bar_out, bar_in, self.G_bar, th, self.advect, k < L - 1,
                                          self._src(s, k - 1) if k > 0 else None,
-                                         self.seg_sort[k - 1] if k > 0 else None, self.stream)
+                                         sort[k - 1] if k > 0 else None, self.stream)
                 bar_out, bar_in = bar_in, bar_out
         theta = self.plan.theta_per_instance(self.theta_acc) if want_theta else None
         return bar_out, self.G_bar, theta

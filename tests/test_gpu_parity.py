@@ -253,3 +253,37 @@ def test_chained_steps_bitwise_equal_single_steps(name, dtype):
         single = T.step_vjp(c, *single, advect=True)
     for a, b in zip(chained, single):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_kept_segments_equal_recompute(dtype):
+    """keep_segments: the forward pass keeps the step records of its last M
+    segments and the reverse pass recomputes only the others.  Final state,
+    every cotangent and the HB parameter cotangents are bit-identical for
+    M = 1 (recompute all but the last), 2 and all segments (no recompute),
+    and stay so over repeated calls on the same trajectory."""
+    from morphompm_amd import constitutive as K
+    from morphompm_amd.engine import BatchPlan, InstanceSpec, Trajectory, ROW_F
+    from morphompm_amd.scenes import free_swell_cube
+    sc = free_swell_cube(cells=5, ppc=8, seed=6, n_steps=11)
+    mat = sc.cfg.material
+    m = K.HerschelBulkley(kappa=mat.lam + 2 * mat.mu / 3, K=10.0, n=0.6)
+    p = BatchPlan([InstanceSpec.from_cfg(sc.n, sc.cfg, m, sc.pts)])
+    dyn0 = p.pack_dyn([(sc.pts, np.zeros((sc.n, 3)), np.zeros((sc.n, 3, 3)),
+                        np.repeat(np.eye(3)[None], sc.n, 0))], dtype)
+    G = p.pack_mat([sc.G], dtype)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    bar = torch.randn(dyn0.shape, dtype=dtype, device="cuda", generator=g) * p.live
+    res = {}
+    for M in (1, 2, 3):                                  # 3 segments of 4, 4, 3 steps
+        tr = Trajectory(p, dtype, sc.n_steps, True, 4, sc.sched, keep_segments=M)
+        out = []
+        for rep in range(2):
+            fin = tr.forward(dyn0, G).clone()
+            b0, Gb, th = tr.backward(bar, want_theta=True)
+            out.append((fin, b0.clone(), Gb.clone(), th.clone(), tr.final().clone()))
+        res[M] = out
+    for M in (2, 3):
+        for rep in range(2):
+            for a, b in zip(res[1][rep], res[M][rep]):
+                assert torch.equal(a, b), (M, rep)
