@@ -108,6 +108,34 @@ __device__ __forceinline__ void st9(T* base, int64_t stride, int64_t p, int row,
   for (int k = 0; k < 9; ++k) base[(int64_t)(row + k) * stride + p] = m.a[k];
 }
 
+// Non-temporal variants for rows a kernel reads or writes once in a stream:
+// they do not displace the grid nodes the same kernel gathers from the caches.
+template <typename T>
+__device__ __forceinline__ void ld3_nt(const T* base, int64_t stride, int64_t p, int row,
+                                       T v[3]) {
+  for (int k = 0; k < 3; ++k)
+    v[k] = __builtin_nontemporal_load(&base[(int64_t)(row + k) * stride + p]);
+}
+template <typename T>
+__device__ __forceinline__ M3<T> ld9_nt(const T* base, int64_t stride, int64_t p, int row) {
+  M3<T> m;
+  for (int k = 0; k < 9; ++k)
+    m.a[k] = __builtin_nontemporal_load(&base[(int64_t)(row + k) * stride + p]);
+  return m;
+}
+template <typename T>
+__device__ __forceinline__ void st3_nt(T* base, int64_t stride, int64_t p, int row,
+                                       const T v[3]) {
+  for (int k = 0; k < 3; ++k)
+    __builtin_nontemporal_store(v[k], &base[(int64_t)(row + k) * stride + p]);
+}
+template <typename T>
+__device__ __forceinline__ void st9_nt(T* base, int64_t stride, int64_t p, int row,
+                                       const M3<T>& m) {
+  for (int k = 0; k < 9; ++k)
+    __builtin_nontemporal_store(m.a[k], &base[(int64_t)(row + k) * stride + p]);
+}
+
 // 4-wide records (grid nodes, payload quads): one vector load per record.
 template <typename T> struct V4 { T x, y, z, w; };
 __device__ __forceinline__ V4<float> ld4(const float* p) {
@@ -885,7 +913,7 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
   if (local < I.n) {
     T x[3], fx[3];
     int base[3];
-    ld3(dyn_in, n_slots, p, RX, x);
+    ld3_nt(dyn_in, n_slots, p, RX, x);
     stencil_base(x, I, base, fx);
     const bool valid = base_in_grid(base, I.nx, I.ny, I.nz);
     // Separable weights: with d = (o - fx) dx,
@@ -927,7 +955,7 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
       }
     };
     if (valid) gather(grid_nodes(grid, I, base));
-    const M3<T> F = ld9(dyn_in, n_slots, p, RF);
+    const M3<T> F = ld9_nt(dyn_in, n_slots, p, RF);
     // F_new = (I + dt C) F, formed as F + dt (C F): I + dt C would round away
     // the low bits of the small increment the same way for every particle.
     const M3<T> CF = mul(nC, F);
@@ -935,12 +963,12 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
     for (int k = 0; k < 9; ++k) Fn.a[k] = F.a[k] + I.dt * CF.a[k];
     T xn[3];
     for (int c = 0; c < 3; ++c) xn[c] = advect ? x[c] + I.dt * nv[c] : x[c];
-    st3(dyn_out, n_slots, p, RX, xn);
-    if (out == MPM_OUT_FULL) st3(dyn_out, n_slots, p, RV, nv);
-    if (out != MPM_OUT_NEXT) st9(dyn_out, n_slots, p, RC, nC);
-    st9(dyn_out, n_slots, p, RF, Fn);
+    st3_nt(dyn_out, n_slots, p, RX, xn);
+    if (out == MPM_OUT_FULL) st3_nt(dyn_out, n_slots, p, RV, nv);
+    if (out != MPM_OUT_NEXT) st9_nt(dyn_out, n_slots, p, RC, nC);
+    st9_nt(dyn_out, n_slots, p, RF, Fn);
     if (NEXT) {
-      const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), sched_next ? sched_next[ii] : T(1));
+      const M3<T> Fg = grown(ld9_nt(Fgp, n_slots, p, 0), sched_next ? sched_next[ii] : T(1));
       int nbase[3];
       make_payload(I, xn, nv, nC, Fn, Fg, payload_next + p * kRecord, nbase);
       next_key = cell_key(inst[ii], nbase, TX, TY, TZ, unit_flag);
