@@ -382,7 +382,8 @@ class Trajectory:
     (M >= the number of segments: no recompute).  Each kept segment costs one
     segment store (K steps of state, grid, sort record and payload: about
     10 GB for 2.7 M particles at K = 20); checkpoints are written every K
-    steps either way.
+    steps either way.  "auto": as many as fit in 60% of the free device
+    memory.
     """
 
     def __init__(self, plan, dtype, n_steps, advect, checkpoint_every=None, sched=None,
@@ -406,6 +407,8 @@ class Trajectory:
         p = plan
         self.ckpt = torch.zeros(max(1, self.nseg), N_DYN, p.n_slots, dtype=dtype, device=p.device)
         Ks = self.K
+        if keep_segments == "auto":
+            keep_segments = self.auto_keep_segments(plan, dtype, Ks, max(1, self.nseg))
         self.M = max(1, min(int(keep_segments), max(1, self.nseg)))
         self._stores = [(torch.zeros(Ks, N_DYN, p.n_slots, dtype=dtype, device=p.device),
                          torch.zeros(Ks, p.n_nodes, 4, dtype=dtype, device=p.device),
@@ -421,6 +424,20 @@ class Trajectory:
         self.theta_acc = torch.zeros(p.n_slots // SLOT_ALIGN, 2, dtype=torch.float64,
                                      device=p.device)
         self._held = [-1] * self.M     # segment whose step records each store holds
+
+    @staticmethod
+    def store_bytes(plan, dtype, K):
+        """Bytes of one segment store (K steps of state, grid, sort record and
+        payload)."""
+        e = torch.tensor([], dtype=dtype).element_size()
+        return K * (plan.n_slots * (N_DYN + 16) * e + plan.n_nodes * 4 * e + plan.sortlen * 4)
+
+    @staticmethod
+    def auto_keep_segments(plan, dtype, K, nseg, fraction=0.6):
+        """As many segment stores as fit in ``fraction`` of the device's free
+        memory (at least 1, at most nseg)."""
+        free, _ = torch.cuda.mem_get_info(plan.device)
+        return max(1, min(nseg, int(fraction * free // Trajectory.store_bytes(plan, dtype, K))))
 
     # segment-store bookkeeping (graph replays restore it as the captured run left it)
     def store_state(self):

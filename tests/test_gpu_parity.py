@@ -287,3 +287,26 @@ def test_kept_segments_equal_recompute(dtype):
         for rep in range(2):
             for a, b in zip(res[1][rep], res[M][rep]):
                 assert torch.equal(a, b), (M, rep)
+
+
+def test_auto_keep_segments():
+    """keep_segments="auto" sizes the kept stores from free device memory: a
+    small scene keeps every segment, and the result equals the default's."""
+    from morphompm_amd import constitutive as K
+    from morphompm_amd.engine import BatchPlan, InstanceSpec, Trajectory
+    from morphompm_amd.scenes import free_swell_cube
+    sc = free_swell_cube(cells=4, ppc=8, seed=2, n_steps=9)
+    m = K.NeoHookean(sc.cfg.material)
+    p = BatchPlan([InstanceSpec.from_cfg(sc.n, sc.cfg, m, sc.pts)])
+    dyn0 = p.pack_dyn([(sc.pts, np.zeros((sc.n, 3)), np.zeros((sc.n, 3, 3)),
+                        np.repeat(np.eye(3)[None], sc.n, 0))], torch.float32)
+    G = p.pack_mat([sc.G], torch.float32)
+    a = Trajectory(p, torch.float32, sc.n_steps, True, 3, sc.sched, keep_segments="auto")
+    b = Trajectory(p, torch.float32, sc.n_steps, True, 3, sc.sched)
+    assert a.M == 3 and b.M == 1
+    assert Trajectory.store_bytes(p, torch.float32, 3) == sum(
+        t.numel() * t.element_size() for t in a._stores[0])
+    fa, fb = a.forward(dyn0, G).clone(), b.forward(dyn0, G).clone()
+    bar = torch.ones_like(fa) * p.live
+    assert torch.equal(fa, fb)
+    assert torch.equal(a.backward(bar)[1], b.backward(bar)[1])
