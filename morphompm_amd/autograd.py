@@ -9,7 +9,7 @@ C0, F0, Fg and, for Herschel-Bulkley models, to ``hb_params`` = (tau_y, K).
 """
 import torch
 
-from .engine import ROW_C, ROW_F, ROW_V, ROW_X, BatchPlan, InstanceSpec, Trajectory
+from .engine import ROW_C, ROW_F, ROW_V, ROW_X, BatchPlan, DEFAULT_TILE, InstanceSpec, Trajectory
 
 
 class _Rollout(torch.autograd.Function):
@@ -35,12 +35,19 @@ def _soa(t, n, k):
     return t.reshape(n, k).transpose(0, 1)
 
 
-def _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched, keep_segments=1):
+def _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched, keep_segments=1,
+           tile=None, block=256, Fg=None):
     n = x0.shape[0]
     # positions are stored relative to the cell nearest their mean (FP32
     # keeps more bits of the displacements)
     spec = InstanceSpec.from_cfg(n, cfg, model, x0.detach(), center=True)
-    plan = BatchPlan([spec])
+    if tile == "auto" or block == "auto":
+        from .autotune import autotune
+        G = Fg.detach() if Fg is not None else torch.eye(3, dtype=x0.dtype).expand(n, 3, 3)
+        tile, block = autotune([spec], [x0.detach()], x0.dtype, advect, [G],
+                               tiles=None if tile == "auto" else [tile or DEFAULT_TILE],
+                               blocks=(64, 128, 256) if block == "auto" else (block,)).choice
+    plan = BatchPlan([spec], tile=tile or DEFAULT_TILE, block=block)
     traj = Trajectory(plan, x0.dtype, n_steps, advect, checkpoint_every, sched,
                       keep_segments=keep_segments)
     return plan, traj
@@ -70,12 +77,15 @@ def _check_hb(model, hb_params):
 
 
 def rollout(x0, v0, C0, F0, Fg, model, cfg, n_steps, advect=False, *, checkpoint_every=None,
-            sched=None, hb_params=None):
+            sched=None, hb_params=None, keep_segments=1, tile=None, block=256):
     """Final (x, v, C, F) after n_steps; differentiable.  Inputs are CUDA
     tensors (n,3)/(n,3,3) of one dtype (float32 or float64).  The plan and
-    its buffers are private to the call."""
+    its buffers are private to the call.  ``tile`` / ``block``: the P2G tile
+    and particle block size (None: the default tile; "auto": the autotuner,
+    cached per shape class); ``keep_segments``: see ``engine.Trajectory``."""
     _check_hb(model, hb_params)
-    plan, traj = _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched)
+    plan, traj = _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched,
+                        keep_segments, tile, block, Fg)
     dyn0, G = _pack(plan, x0, v0, C0, F0, Fg)
     fin = _Rollout.apply(dyn0, G, hb_params, traj, plan, plan.specs[0].prm)
     return _unpack(plan, fin, x0.shape[0], x0.dtype)
@@ -104,15 +114,17 @@ class GraphRollout:
     fixed ones.  Inputs are copied into the runner's own buffers; outputs and
     gradients are fresh tensors.  One runner serves one call at a time.
     ``keep_segments``: step-record stores kept from the forward pass (see
-    ``engine.Trajectory``); more trade HBM for less recompute."""
+    ``engine.Trajectory``); more trade HBM for less recompute.  ``tile`` /
+    ``block`` as for ``rollout`` (``Fg``: a growth tensor the autotuner may
+    time with)."""
 
     def __init__(self, x0, model, cfg, n_steps, advect=False, *, checkpoint_every=None,
-                 sched=None, hb=False, keep_segments=1):
+                 sched=None, hb=False, keep_segments=1, tile=None, block=256, Fg=None):
         if hb and getattr(model, "kind", None) != "hb":
             raise ValueError("hb applies to HerschelBulkley models only")
         self.model, self.n, self.dtype, self.hb = model, x0.shape[0], x0.dtype, bool(hb)
         self.plan, self.traj = _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched,
-                                      keep_segments)
+                                      keep_segments, tile, block, Fg)
         p = self.plan
         self.prm_base = list(p.specs[0].prm)
         self.dyn_in = p.new_dyn(self.dtype)

@@ -113,3 +113,20 @@ def test_graph_rollout_replays_match_eager(kind, dtype):
         torch.cuda.synchronize()
         for i, (a, b) in enumerate(zip(ref, got)):
             assert torch.equal(a, b), (rep, i)
+
+
+def test_rollout_tile_choice_fp64(tmp_path, monkeypatch):
+    """rollout's tile/block choice (an explicit tile, or the autotuner's)
+    changes only the summation order: FP64 results agree to 1e-11 of each
+    output's largest entry."""
+    from morphompm_amd import autograd as AG, constitutive as K
+    from morphompm_amd.scenes import free_swell_cube
+    monkeypatch.setenv("MORPHOMPM_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    sc = free_swell_cube(cells=5, ppc=8, seed=1)
+    m = K.Hencky(sc.cfg.material)
+    x0, v0, C0, F0, Fg = _inputs(sc.pts, 1.08, torch.float64)
+    ref = AG.rollout(x0, v0, C0, F0, Fg, m, sc.cfg, 6, advect=True, checkpoint_every=3)
+    for kw in (dict(tile=(4, 8, 8), block=128), dict(tile="auto", block="auto")):
+        got = AG.rollout(x0, v0, C0, F0, Fg, m, sc.cfg, 6, advect=True, checkpoint_every=3, **kw)
+        for a, b in zip(ref, got):     # relative to each array's largest entry
+            assert float((a - b).abs().max() / b.abs().max()) <= 1e-11, kw
