@@ -159,6 +159,18 @@ def _cpu_loop_worker(args):
     return n, time.perf_counter() - t0
 
 
+def _cpu_model():
+    """The host CPU's model name, as /proc/cpuinfo reports it."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scenes, budget_s):
     """FP64 vectorised numpy oracle on the host: P single-thread processes over
     independent instances, each run for a truncated step count."""
@@ -185,6 +197,7 @@ def cpu_baseline(scenes, budget_s):
     small = min(jobs, key=lambda j: len(j[0]))
     n_loop, t_loop = _cpu_loop_worker(small)
     return {"value": done / wall, "unit": "particle-steps/s", "cores": P, "kind": "port",
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
             "loop_faithful_single_thread_value": n_loop / t_loop,
             "loop_faithful_sample": f"1 fwd+adjoint step of a {n_loop}-particle instance "
                                     f"in {t_loop:.1f} s (the reference's loop structure)",
