@@ -813,12 +813,22 @@ __device__ __forceinline__ NodeBox chunk_box(bool valid, const int base[3], int*
   return b;
 }
 
+// n / d for 0 <= n < 2^20, given inv_d = 1 / d: (n + 1/2) / d lies at
+// least 1 / (2d) from an integer, and the float product is within
+// (n + 1/2) / d * 2^-23 of it.
+__device__ __forceinline__ int small_div(int n, float inv_d) {
+  return (int)(((float)n + 0.5f) * inv_d);
+}
+
 // lds[n] = record of box node n (x-major), zero outside the grid.
 template <typename T>
 __device__ __forceinline__ void stage_box(const Inst<T>& I, const NodeBox& b, const T* rec,
                                           V4<T>* lds) {
+  const int plane = b.d[1] * b.d[2];  // <= b.vol <= the box capacity
+  const float inv_plane = 1.0f / (float)plane, inv_d2 = 1.0f / (float)b.d[2];
   for (int n = threadIdx.x; n < b.vol; n += 256) {
-    const int k = n % b.d[2], j = (n / b.d[2]) % b.d[1], i = n / (b.d[1] * b.d[2]);
+    const int i = small_div(n, inv_plane), r = n - i * plane;
+    const int j = small_div(r, inv_d2), k = r - j * b.d[2];
     const int gx = b.lo[0] + i, gy = b.lo[1] + j, gz = b.lo[2] + k;
     V4<T> g = {T(0), T(0), T(0), T(0)};
     if (gx >= 0 && gx < I.nx && gy >= 0 && gy < I.ny && gz >= 0 && gz < I.nz)
