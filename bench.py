@@ -57,15 +57,17 @@ def g2p_bytes_per_particle(n_steps, K):
     return total / launches
 
 
-def adj_vjp_bytes_per_particle(n_steps, K):
-    """Average algorithmic bytes per particle of one constitutive-VJP launch
-    over an iteration: MD, C, F, G, C_new, Fn_b and G_b in, F_b and G_b out
-    (256 + 108 B); a launch that also prepares the previous step's adjoint
-    payload reads x, F of that step, x_b, v_b and its rank (76 B) and writes
-    the 64 B payload instead of the C_b rows (36 B): 468 B."""
+def adj_particle_bytes_per_particle(n_steps, K):
+    """Average algorithmic bytes per particle of one per-particle reverse
+    launch over an iteration (P2G-adjoint gather and constitutive VJP): rank,
+    the forward payload (64 B), the adjoint payload (64 B), xn_b, G, C, F,
+    C_new, Fn_b and G_b in (380 B); x_b, v_b, F_b and G_b out (96 B).  A
+    launch that also prepares the previous step's adjoint payload reads x, F
+    of that step and its rank (52 B) and writes the 64 B payload; the first
+    step of a segment writes the C_b rows (36 B) instead: 592 and 512 B."""
     nseg = -(-n_steps // K)
     seg_len = [min(K, n_steps - s * K) for s in range(nseg)]
-    total = sum((L - 1) * 468 + 364 for L in seg_len)
+    total = sum((L - 1) * 592 + 512 for L in seg_len)
     return total / n_steps
 
 
@@ -80,13 +82,11 @@ KERNELS = {
     5: ("prep_adj", lambda n, a, nb: 208 * n),              # x F + 4 cotangents in, payload out
     6: ("p2g_adj_tile", lambda n, a, nb: 64 * n + 16 * a),
     7: ("grid_adj", lambda n, a, nb: 48 * a),               # partials + grid in, adjoint grid out
-    8: ("adj_gather", lambda n, a, nb: 204 * n + 32 * a),   # sorted, 2 payloads, xn_b, nodes in;
-                                                             # x_b v_b MD out
-    9: ("adj_vjp", lambda n, a, nb: ADJ_VJP_BYTES * n),     # see adj_vjp_bytes_per_particle
+    8: ("adj_particle", lambda n, a, nb: ADJ_PARTICLE_BYTES * n + 32 * a),  # see
+                                                             # adj_particle_bytes_per_particle
 }
-ADJ_KERNELS = {5, 6, 7, 8, 9}
 G2P_BYTES = 248.0   # set from the trajectory shape in main()
-ADJ_VJP_BYTES = 468.0
+ADJ_PARTICLE_BYTES = 580.0
 
 
 def parse():
@@ -228,9 +228,9 @@ def main():
     for s in scenes:
         s.checkpoint_every = a.checkpoint_every
     n_total = sum(s.n for s in scenes)
-    global G2P_BYTES, ADJ_VJP_BYTES
+    global G2P_BYTES, ADJ_PARTICLE_BYTES
     G2P_BYTES = g2p_bytes_per_particle(a.sim_steps, a.checkpoint_every)
-    ADJ_VJP_BYTES = adj_vjp_bytes_per_particle(a.sim_steps, a.checkpoint_every)
+    ADJ_PARTICLE_BYTES = adj_particle_bytes_per_particle(a.sim_steps, a.checkpoint_every)
     rates = np.array([s.extra["rate"] for s in scenes])
     print(f"[bench] {len(scenes)} instances, {n_total} particles, {a.sim_steps} steps, "
           f"K={a.checkpoint_every}", file=sys.stderr, flush=True)

@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 5
+#define MPM_ABI_VERSION 6
 #define MPM_SLOT_ALIGN 256 /* particle and node ranges of an instance start at multiples of this */
 
 enum { MPM_OK = 0, MPM_EINVAL = -1, MPM_ELAUNCH = -2, MPM_ENOTSUP = -3 };
@@ -201,14 +201,17 @@ int32_t mpm_step_bwd_f64(const mpm_batch* b, const double* dyn_in, const double*
                          void* ws, void* stream);
 
 /* The same reverse step for a chain of reverse steps on one workspace and
- * stream (steps t, t-1, ... of one checkpoint segment).  ready = 1: this
- * step's G2P-adjoint payload is in ws from the previous chained call (that of
- * step t+1), which must be the last call on this workspace; ready = 0: it is
- * computed from dyn_in and bar_out first.  dyn_prev / sortbuf_prev (both or
- * NULL): step t-1's input state and sort record; the constitutive-VJP kernel
- * then also writes step t-1's payload into ws, which carries C_bar, and the C
- * rows of bar_in are not written (the chain reads them from nowhere else; a
- * caller that needs C_bar of an intermediate step passes NULL).  Results are
+ * stream (steps t, t-1, ... of one checkpoint segment).  ready = 0: this
+ * step's G2P-adjoint payload is computed from dyn_in and bar_out first (the
+ * first call of a chain); ready = 1 or 2: it is in ws from the previous
+ * chained call (that of step t+1), which must be the last call on this
+ * workspace -- the i-th call of a chain (i = 0, 1, ...) passes ready = 0 for
+ * i = 0, else 1 + (i % 2): the workspace holds two payload areas and each
+ * call reads one and writes the other.  dyn_prev / sortbuf_prev (both or
+ * NULL): step t-1's input state and sort record; the call then also writes
+ * step t-1's payload into ws, which carries C_bar, and the C rows of bar_in
+ * are not written (the chain reads them from nowhere else; a caller that
+ * needs C_bar of an intermediate step passes NULL).  Results are
  * bit for bit those of mpm_step_bwd in the rows written.  No reference
  * counterpart (a fusion of consecutive transfer.step_vjp calls,
  * python/morphompm/integrate.py:24-28). */
@@ -243,8 +246,8 @@ enum {
   MPM_K_PREP_ADJ = 5,
   MPM_K_P2G_ADJ_TILE = 6, /* G2P-adjoint scatter as a tile reduction */
   MPM_K_GRID_ADJ = 7,
-  MPM_K_P2G_ADJ = 8,     /* P2G-adjoint gather (x, v cotangents, MD) */
-  MPM_K_ADJ_VJP = 9      /* constitutive VJP, F, C, Fg cotangents */
+  MPM_K_ADJ_PARTICLE = 8 /* per particle: P2G-adjoint gather (x, v cotangents) and the
+                            constitutive VJP (F, C, Fg cotangents) */
 };
 int32_t mpm_kernel_timer(int32_t kernel_id, void* const* events, int32_t n_pairs);
 int32_t mpm_kernel_timer_count(void);

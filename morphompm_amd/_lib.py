@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # to compare two builds on one device; the in-tree build is the default.
 LIB_PATH = os.environ.get("MORPHOMPM_AMD_LIB") or os.path.join(HERE, "_lib", "libmorphompm_hip.so")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 SLOT_ALIGN = 256
 
 NEO_HOOKEAN, HENCKY, HERSCHEL_BULKLEY = 0, 1, 2

@@ -18,8 +18,9 @@
 //   prep_adj : G2P-adjoint payload (nv_bar, C_bar') in sorted order;
 //   p2g tile : the G2P-adjoint scatter as the same segmented reduction;
 //   grid_adj : gmom_bar = s gv_bar / m, gm_bar = -(gv_bar . gv) / m;
-//   p2g_adj  : 27-node gather of (gmom_bar, gm_bar, gv), advection terms, and
-//              the constitutive VJP.
+//   adj      : per particle, in sorted order: 27-node gather of (gmom_bar,
+//              gm_bar, gv) and the advection terms, then the constitutive VJP,
+//              with the gathered values kept in registers.
 // Every sum has a fixed order, so results are bitwise reproducible.
 #include <hip/hip_runtime.h>
 
@@ -1055,31 +1056,42 @@ __global__ __launch_bounds__(256) void k_grid_adj(const mpm_instance* inst, cons
   st4(gadj + 4 * g, gmb, mb[0], mb[1], mb[2]);
 }
 
-// reverse step, part 3a (transfer.py:100-119 x_bar terms, :129-147): the
-// P2G-adjoint gather, one thread per sorted slot j; each 256-slot chunk
-// stages its stencil box of the grid adjoint (and, with advection, of the
-// grid) in LDS.  Reads the forward payload pf (fx, q2, A', cell) of particle
-// p = sorted[j] and the adjoint payload pa (fx, q1, P1) of j.  With dpos = (o - fx) dx for
+// reverse step, part 3 (transfer.py:100-119 x_bar terms, :129-156), one
+// thread per particle slot p (coalesced state rows), at sorted position
+// j = rank[p]; each 256-slot chunk stages its stencil box of the grid adjoint
+// (and, with advection, of the grid) in LDS.
+//
+// The P2G-adjoint gather reads the forward payload pf (fx, q2, A', cell) of p
+// and the adjoint payload pa (fx, q1, P1) at j.  With dpos = (o - fx) dx for
 // stencil offset o, the per-node factors are affine in o:
 //   nv_b + 4 inv_dx^2 C_b' dpos = q1 + P1 o   (P1 = 4 inv_dx^2 dx C_b')
 //   pmass v + A dpos            = q2 + A' o   (A' = A dx, A = pmass C + dt c tau)
 // so x_bar's grad-w coefficient is v_i.(q1 + P1 o) + mb_i.(q2 + A' o) + gm_b,i pmass.
-// Writes x_bar, v_bar of the particle and MD = sum w mb dpos^T (row-major, at
-// the sorted position j; the VJP kernel reads it through rank[p]).
-template <typename T>
-__global__ __launch_bounds__(256) void k_adj_gather(const mpm_instance* inst,
-                                                    const int32_t* slot_inst, int64_t n_slots,
-                                                    const int32_t* sorted, const T* pf, const T* pa,
-                                                    const T* grid, const T* gadj, const T* bar_out,
-                                                    T* bar_in, T* md, int advect) {
+// It gives x_bar, v_bar and MD = sum w mb dpos^T.
+//
+// The constitutive VJP then follows in the same thread, with MD in registers:
+// tau_b = c sk_b with sk_b = dt MD, c = -4 inv_dx^2 vol0 det Fg;
+//   C_bar = pmass MD + C_bar(stress),  F_bar = (I + dt Cn)^T Fn_b + F_bar(stress),
+//   G_bar += a (Fg_bar(stress) + <sk_b, tau> (-4 inv_dx^2 vol0) det Fg Fg^-T).
+// PREV: with the state's full cotangent at hand, also writes the previous
+// step's G2P-adjoint payload, what k_prep_adj computes for that step, at its
+// sorted position rank_prev[p]; C_bar then goes only there.
+template <typename T, int MODEL, bool PREV>
+__global__ __launch_bounds__(256) void k_adj_particle(
+    const mpm_instance* inst, const int32_t* slot_inst, int64_t n_slots, const int32_t* rank,
+    const T* pf, const T* pa, const T* grid, const T* gadj, const T* dyn_in, const T* Fgp,
+    const T* sched, const T* dyn_out, const T* bar_out, T* bar_in, T* Fg_bar, double* theta_acc,
+    const T* dyn_prev, const int32_t* rank_prev, T* payload_prev, int advect,
+    const int32_t* chunks) {
   constexpr int CAP = BoxCap<T>::value;
   __shared__ V4<T> lg[CAP];  // grid adjoint (gm_b, mb)
   __shared__ V4<T> lv[CAP];  // grid (m, v), advection only
   __shared__ int red[24];
-  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;  // n_slots is a multiple of 256
-  const Inst<T> I = load_inst<T>(inst, block_inst(slot_inst));
-  const int64_t p = sorted[j];
+  const int64_t p0 = block_first_slot(chunks), p = p0 + threadIdx.x;  // blockDim = 256
+  const int ii = __builtin_amdgcn_readfirstlane(slot_inst[p0 / kChunk]);
+  const Inst<T> I = load_inst<T, MODEL>(inst, ii);
   const bool live = p - I.p_off < I.n;
+  const int64_t j = live ? rank[p] : 0;
   T fx[3] = {T(0), T(0), T(0)}, q1[3], q2[3];
   M3<T> P1, Ap;
   uint32_t cell = kNoCell;
@@ -1106,121 +1118,98 @@ __global__ __launch_bounds__(256) void k_adj_gather(const mpm_instance* inst,
     if (advect) stage_box(I, bx, grid, lv);
   }
   __syncthreads();
-  if (!live) return;
-  T xb[3] = {T(0), T(0), T(0)};
-  T vb[3] = {T(0), T(0), T(0)};
-  M3<T> MD = zero3<T>();
-  T Sv[3] = {T(0), T(0), T(0)};  // sum w v_i
-  // Separable weights: with d = (o - fx) dx, vb = sum w mb and
-  //   MD = dx (sum w mb o^T - vb fx^T);
-  // x_bar's terms sum alpha grad w, grad w = (dwx wy wz, wx dwy wz, wx wy dwz) / dx,
-  // over each z-row (i, j) of 3 nodes first.
-  M3<T> Mo = zero3<T>();  // sum w mb o^T
-  auto gather = [&](const auto& adj_nodes, const auto& vel_nodes) {
-    T w[3][3], dw[3][3];
-    for (int c = 0; c < 3; ++c) {
-      bspline_w(fx[c], w[c]);
-      bspline_dw(fx[c], dw[c]);
-    }
-    const T wz0 = w[2][0], wz1 = w[2][1], wz2 = w[2][2], wz2k = T(2) * w[2][2];
-    // The (i, j) loop stays rolled (9 rows of 3 nodes) to bound the live
-    // values; per-axis values of row (i, j) are picked by select.
-#pragma unroll 1
-    for (int ij = 0; ij < 9; ++ij) {
-      const int i = ij / 3, jj = ij - 3 * (ij / 3);
-      const T wxi = i == 0 ? w[0][0] : (i == 1 ? w[0][1] : w[0][2]);
-      const T dxi = i == 0 ? dw[0][0] : (i == 1 ? dw[0][1] : dw[0][2]);
-      const T wyj = jj == 0 ? w[1][0] : (jj == 1 ? w[1][1] : w[1][2]);
-      const T dyj = jj == 0 ? dw[1][0] : (jj == 1 ? dw[1][1] : dw[1][2]);
-      const T Ti = T(i), Tj = T(jj);
-      const T wxy = wxi * wyj;
-      V4<T> ga[3];
-      for (int k = 0; k < 3; ++k) ga[k] = adj_nodes(i, jj, k);
-      for (int c = 0; c < 3; ++c) {
-        const T m0 = c == 0 ? ga[0].y : (c == 1 ? ga[0].z : ga[0].w);
-        const T m1 = c == 0 ? ga[1].y : (c == 1 ? ga[1].z : ga[1].w);
-        const T m2 = c == 0 ? ga[2].y : (c == 1 ? ga[2].z : ga[2].w);
-        const T r0 = wz0 * m0 + wz1 * m1 + wz2 * m2;
-        const T r1 = wz1 * m1 + wz2k * m2;
-        vb[c] += wxy * r0;
-        Mo(c, 0) += (Ti * wxy) * r0;
-        Mo(c, 1) += (Tj * wxy) * r0;
-        Mo(c, 2) += wxy * r1;
-      }
-      if (advect) {
-        T u1[3], u2[3];
-        for (int c = 0; c < 3; ++c) {
-          u1[c] = q1[c] + Ti * P1(c, 0) + Tj * P1(c, 1);
-          u2[c] = q2[c] + Ti * Ap(c, 0) + Tj * Ap(c, 1);
-        }
-        T A0 = T(0), A1 = T(0);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const V4<T> gv = vel_nodes(i, jj, k);
-          const T vi[3] = {gv.y, gv.z, gv.w};
-          const T mb[3] = {ga[k].y, ga[k].z, ga[k].w};
-          T alpha = ga[k].x * I.pmass;
-          for (int c = 0; c < 3; ++c)
-            alpha += vi[c] * (u1[c] + T(k) * P1(c, 2)) + mb[c] * (u2[c] + T(k) * Ap(c, 2));
-          A0 += w[2][k] * alpha;
-          A1 += dw[2][k] * alpha;
-          const T wt = wxy * w[2][k];
-          Sv[0] += wt * gv.y;
-          Sv[1] += wt * gv.z;
-          Sv[2] += wt * gv.w;
-        }
-        xb[0] += (dxi * wyj * I.inv_dx) * A0;
-        xb[1] += (wxi * dyj * I.inv_dx) * A0;
-        xb[2] += (wxy * I.inv_dx) * A1;
-      }
-    }
-  };
-  if (valid) {
-    if (bx.staged)
-      gather(box_nodes(lg, bx, base), box_nodes(lv, bx, base));
-    else
-      gather(grid_nodes(gadj, I, base), grid_nodes(grid, I, base));
-  }
-  if (advect) {
-    // x_bar += xn_b - (4 inv_dx^2 C_b')^T (sum w v_i) - A^T (sum w mb_i), A = A'/dx
-    T xnb[3];
-    ld3(bar_out, n_slots, p, RX, xnb);
-    for (int c = 0; c < 3; ++c)
-      xb[c] += xnb[c] - ((P1(0, c) * Sv[0] + P1(1, c) * Sv[1] + P1(2, c) * Sv[2]) +
-                         (Ap(0, c) * vb[0] + Ap(1, c) * vb[1] + Ap(2, c) * vb[2])) * I.inv_dx;
-  }
-  for (int c = 0; c < 3; ++c)
-    for (int e = 0; e < 3; ++e) MD(c, e) = I.dx * (Mo(c, e) - vb[c] * fx[e]);
-  for (int c = 0; c < 3; ++c) vb[c] *= I.pmass;
-  st3(bar_in, n_slots, p, RX, xb);
-  st3(bar_in, n_slots, p, RV, vb);
-  st9(md, n_slots, j, 0, MD);
-}
-
-// reverse step, part 3b (transfer.py:148-156): constitutive VJP, one thread
-// per particle slot.  tau_b = c sk_b with sk_b = dt MD, c = -4 inv_dx^2 vol0 det Fg;
-//   C_bar = pmass MD + C_bar(stress),  F_bar = (I + dt Cn)^T Fn_b + F_bar(stress),
-//   G_bar += a (Fg_bar(stress) + <sk_b, tau> (-4 inv_dx^2 vol0) det Fg Fg^-T).
-// PREV: with the state's full cotangent at hand (x_bar, v_bar from the gather
-// kernel), also writes the previous step's G2P-adjoint payload, what
-// k_prep_adj computes for that step, at its sorted position rank_prev[p].
-template <typename T, int MODEL, bool PREV>
-__global__ __launch_bounds__(256) void k_adj_vjp(const mpm_instance* inst, const int32_t* slot_inst,
-                                                 int64_t n_slots, const int32_t* rank,
-                                                 const T* dyn_in, const T* Fgp, const T* sched,
-                                                 const T* dyn_out, const T* md, const T* bar_out,
-                                                 T* bar_in, T* Fg_bar, double* theta_acc,
-                                                 const T* dyn_prev, const int32_t* rank_prev,
-                                                 T* payload_prev, int advect,
-                                                 const int32_t* chunks) {
-  __shared__ double red[2][256];
-  const int64_t p0 = block_first_slot(chunks), p = p0 + threadIdx.x;  // blockDim = 256
-  const int ii = __builtin_amdgcn_readfirstlane(slot_inst[p0 / kChunk]);
-  const Inst<T> I = load_inst<T, MODEL>(inst, ii);
-  const bool live = (p < n_slots) && (p - I.p_off < I.n);
   T th[2] = {T(0), T(0)};
   if (live) {
-    const M3<T> MD = ld9(md, n_slots, rank[p], 0);
+    T xb[3] = {T(0), T(0), T(0)};
+    T vb[3] = {T(0), T(0), T(0)};
+    M3<T> MD = zero3<T>();
+    {
+      T Sv[3] = {T(0), T(0), T(0)};  // sum w v_i
+      // Separable weights: with d = (o - fx) dx, vb = sum w mb and
+      //   MD = dx (sum w mb o^T - vb fx^T);
+      // x_bar's terms sum alpha grad w, grad w = (dwx wy wz, wx dwy wz, wx wy dwz) / dx,
+      // over each z-row (i, j) of 3 nodes first.
+      M3<T> Mo = zero3<T>();  // sum w mb o^T
+      auto gather = [&](const auto& adj_nodes, const auto& vel_nodes) {
+        T w[3][3], dw[3][3];
+        for (int c = 0; c < 3; ++c) {
+          bspline_w(fx[c], w[c]);
+          bspline_dw(fx[c], dw[c]);
+        }
+        const T wz0 = w[2][0], wz1 = w[2][1], wz2 = w[2][2], wz2k = T(2) * w[2][2];
+        // The (i, j) loop stays rolled (9 rows of 3 nodes) to bound the live
+        // values; per-axis values of row (i, j) are picked by select.
+#pragma unroll 1
+        for (int ij = 0; ij < 9; ++ij) {
+          const int i = ij / 3, jj = ij - 3 * (ij / 3);
+          const T wxi = i == 0 ? w[0][0] : (i == 1 ? w[0][1] : w[0][2]);
+          const T dxi = i == 0 ? dw[0][0] : (i == 1 ? dw[0][1] : dw[0][2]);
+          const T wyj = jj == 0 ? w[1][0] : (jj == 1 ? w[1][1] : w[1][2]);
+          const T dyj = jj == 0 ? dw[1][0] : (jj == 1 ? dw[1][1] : dw[1][2]);
+          const T Ti = T(i), Tj = T(jj);
+          const T wxy = wxi * wyj;
+          V4<T> ga[3];
+          for (int k = 0; k < 3; ++k) ga[k] = adj_nodes(i, jj, k);
+          for (int c = 0; c < 3; ++c) {
+            const T m0 = c == 0 ? ga[0].y : (c == 1 ? ga[0].z : ga[0].w);
+            const T m1 = c == 0 ? ga[1].y : (c == 1 ? ga[1].z : ga[1].w);
+            const T m2 = c == 0 ? ga[2].y : (c == 1 ? ga[2].z : ga[2].w);
+            const T r0 = wz0 * m0 + wz1 * m1 + wz2 * m2;
+            const T r1 = wz1 * m1 + wz2k * m2;
+            vb[c] += wxy * r0;
+            Mo(c, 0) += (Ti * wxy) * r0;
+            Mo(c, 1) += (Tj * wxy) * r0;
+            Mo(c, 2) += wxy * r1;
+          }
+          if (advect) {
+            T u1[3], u2[3];
+            for (int c = 0; c < 3; ++c) {
+              u1[c] = q1[c] + Ti * P1(c, 0) + Tj * P1(c, 1);
+              u2[c] = q2[c] + Ti * Ap(c, 0) + Tj * Ap(c, 1);
+            }
+            T A0 = T(0), A1 = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const V4<T> gv = vel_nodes(i, jj, k);
+              const T vi[3] = {gv.y, gv.z, gv.w};
+              const T mb[3] = {ga[k].y, ga[k].z, ga[k].w};
+              T alpha = ga[k].x * I.pmass;
+              for (int c = 0; c < 3; ++c)
+                alpha += vi[c] * (u1[c] + T(k) * P1(c, 2)) + mb[c] * (u2[c] + T(k) * Ap(c, 2));
+              A0 += w[2][k] * alpha;
+              A1 += dw[2][k] * alpha;
+              const T wt = wxy * w[2][k];
+              Sv[0] += wt * gv.y;
+              Sv[1] += wt * gv.z;
+              Sv[2] += wt * gv.w;
+            }
+            xb[0] += (dxi * wyj * I.inv_dx) * A0;
+            xb[1] += (wxi * dyj * I.inv_dx) * A0;
+            xb[2] += (wxy * I.inv_dx) * A1;
+          }
+        }
+      };
+      if (valid) {
+        if (bx.staged)
+          gather(box_nodes(lg, bx, base), box_nodes(lv, bx, base));
+        else
+          gather(grid_nodes(gadj, I, base), grid_nodes(grid, I, base));
+      }
+      if (advect) {
+        // x_bar += xn_b - (4 inv_dx^2 C_b')^T (sum w v_i) - A^T (sum w mb_i), A = A'/dx
+        T xnb[3];
+        ld3(bar_out, n_slots, p, RX, xnb);
+        for (int c = 0; c < 3; ++c)
+          xb[c] += xnb[c] - ((P1(0, c) * Sv[0] + P1(1, c) * Sv[1] + P1(2, c) * Sv[2]) +
+                             (Ap(0, c) * vb[0] + Ap(1, c) * vb[1] + Ap(2, c) * vb[2])) * I.inv_dx;
+      }
+      for (int c = 0; c < 3; ++c)
+        for (int e = 0; e < 3; ++e) MD(c, e) = I.dx * (Mo(c, e) - vb[c] * fx[e]);
+      for (int c = 0; c < 3; ++c) vb[c] *= I.pmass;
+    }
+    st3(bar_in, n_slots, p, RX, xb);
+    st3(bar_in, n_slots, p, RV, vb);
+
     const T a_t = sched ? sched[ii] : T(1);
     const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), a_t);
     const T detG = det(Fg);
@@ -1246,10 +1235,8 @@ __global__ __launch_bounds__(256) void k_adj_vjp(const mpm_instance* inst, const
       st9(bar_in, n_slots, p, RF, Fpb);
     }
     if (PREV) {
-      T xp[3], xb[3], vb[3];
+      T xp[3];
       ld3(dyn_prev, n_slots, p, RX, xp);
-      ld3(bar_in, n_slots, p, RX, xb);
-      ld3(bar_in, n_slots, p, RV, vb);
       make_adj_payload(I, xp, ld9(dyn_prev, n_slots, p, RF), xb, vb, Cbo, Fpb, advect,
                        payload_prev + (int64_t)rank_prev[p] * kRecord);
     }
@@ -1260,19 +1247,20 @@ __global__ __launch_bounds__(256) void k_adj_vjp(const mpm_instance* inst, const
     st9(Fg_bar, n_slots, p, 0, Gb);
   }
   if (theta_acc != nullptr) {  // fixed-order block reduction, one slot per 256 particles
-    red[0][threadIdx.x] = (double)th[0];
-    red[1][threadIdx.x] = (double)th[1];
+    __shared__ double tred[2][256];
+    tred[0][threadIdx.x] = (double)th[0];
+    tred[1][threadIdx.x] = (double)th[1];
     __syncthreads();
     for (int off = blockDim.x / 2; off > 0; off >>= 1) {
       if (threadIdx.x < (unsigned)off) {
-        red[0][threadIdx.x] += red[0][threadIdx.x + off];
-        red[1][threadIdx.x] += red[1][threadIdx.x + off];
+        tred[0][threadIdx.x] += tred[0][threadIdx.x + off];
+        tred[1][threadIdx.x] += tred[1][threadIdx.x + off];
       }
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      theta_acc[2 * (p0 / kChunk)] += red[0][0];
-      theta_acc[2 * (p0 / kChunk) + 1] += red[1][0];
+      theta_acc[2 * (p0 / kChunk)] += tred[0][0];
+      theta_acc[2 * (p0 / kChunk) + 1] += tred[1][0];
     }
   }
 }
@@ -1331,9 +1319,11 @@ int64_t tile_nodes(const mpm_batch* b) {
 
 inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
+// payload and payload2: the two areas of the G2P-adjoint payload in a chain
+// of reverse steps, each step reading one and writing the other.
 struct Workspace {
   int32_t *key, *cursor, *placed, *count, *bsum, *unit_flag, *ucount;
-  void *payload, *partial, *gadj, *md;
+  void *payload, *payload2, *partial, *gadj;
 };
 
 // Views into a step's sort record:
@@ -1374,7 +1364,7 @@ int64_t carve(const mpm_batch* b, size_t tsz, char* base, Workspace* w) {
   ws.payload = take((int64_t)tsz * kRecord * b->n_slots);
   ws.partial = take((int64_t)tsz * 4 * tile_nodes(b) * b->n_tiles);
   ws.gadj = take((int64_t)tsz * 4 * b->n_nodes);
-  ws.md = take((int64_t)tsz * 9 * b->n_slots);
+  ws.payload2 = take((int64_t)tsz * kRecord * b->n_slots);
   if (w) *w = ws;
   return off;
 }
@@ -1533,12 +1523,11 @@ void launch_tile_fwd(const mpm_batch* b, const SortView<const int32_t>& v, const
 
 template <typename T, int TX, int TY, int TZ>
 void launch_tile_adj(const mpm_batch* b, const SortView<const int32_t>& v, const Workspace& w,
-                     const T* grid, int advect, hipStream_t st) {
+                     const T* pa, const T* grid, int advect, hipStream_t st) {
   {
   TimedLaunch tl(MPM_K_P2G_ADJ_TILE, st);
   k_p2g_tile<T, TX, TY, TZ, false, false><<<dim3((unsigned)b->n_tiles), TX * TY * TZ, 0, st>>>(
-      b->inst, b->tile_inst, v.active_unit, v.n_active, v.start, v.sorted, (const T*)w.payload,
-      (T*)w.partial);
+      b->inst, b->tile_inst, v.active_unit, v.n_active, v.start, v.sorted, pa, (T*)w.partial);
   }
   {
   TimedLaunch tl(MPM_K_GRID_ADJ, st);
@@ -1563,14 +1552,14 @@ void dispatch_fwd(int ti, const mpm_batch* b, const SortView<const int32_t>& sv,
 
 template <typename T>
 void dispatch_adj(int ti, const mpm_batch* b, const SortView<const int32_t>& sv,
-                  const Workspace& w, const T* grid, int advect, hipStream_t st) {
+                  const Workspace& w, const T* pa, const T* grid, int advect, hipStream_t st) {
   switch (ti) {
-    case 0: launch_tile_adj<T, 4, 8, 8>(b, sv, w, grid, advect, st); break;
-    case 1: launch_tile_adj<T, 4, 4, 16>(b, sv, w, grid, advect, st); break;
-    case 2: launch_tile_adj<T, 8, 8, 4>(b, sv, w, grid, advect, st); break;
-    case 4: launch_tile_adj<T, 4, 4, 4>(b, sv, w, grid, advect, st); break;
-    case 5: launch_tile_adj<T, 2, 4, 8>(b, sv, w, grid, advect, st); break;
-    default: launch_tile_adj<T, 4, 4, 8>(b, sv, w, grid, advect, st); break;
+    case 0: launch_tile_adj<T, 4, 8, 8>(b, sv, w, pa, grid, advect, st); break;
+    case 1: launch_tile_adj<T, 4, 4, 16>(b, sv, w, pa, grid, advect, st); break;
+    case 2: launch_tile_adj<T, 8, 8, 4>(b, sv, w, pa, grid, advect, st); break;
+    case 4: launch_tile_adj<T, 4, 4, 4>(b, sv, w, pa, grid, advect, st); break;
+    case 5: launch_tile_adj<T, 2, 4, 8>(b, sv, w, pa, grid, advect, st); break;
+    default: launch_tile_adj<T, 4, 4, 8>(b, sv, w, pa, grid, advect, st); break;
   }
 }
 
@@ -1623,11 +1612,13 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   return check_launch("step_fwd");
 }
 
-// One reverse step.  ready: this step's G2P-adjoint payload was written into
-// the workspace by the previous chained reverse call (the later step's);
-// else k_prep_adj computes it.  dyn_prev / sortbuf_prev (both or neither):
-// the previous step's input state and sort record; the VJP kernel then also
-// writes that step's G2P-adjoint payload for the next chained call.
+// One reverse step.  ready = 1 or 2: this step's G2P-adjoint payload was
+// written into payload area ready - 1 of the workspace by the previous chained
+// reverse call (the later step's); ready = 0: k_prep_adj computes it into area
+// 0.  dyn_prev / sortbuf_prev (both or neither): the previous step's input
+// state and sort record; the per-particle kernel then also writes that step's
+// G2P-adjoint payload, into the other area (its blocks still read this step's
+// payload while others write the next one).
 template <typename T>
 int step_bwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sched,
                   const T* dyn_out, const T* grid, const int32_t* sortbuf, const T* payload,
@@ -1641,36 +1632,34 @@ int step_bwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   if (bar_in == bar_out) return set_err(MPM_EINVAL, "step_bwd: bar_in must not alias bar_out");
   if ((dyn_prev == nullptr) != (sortbuf_prev == nullptr))
     return set_err(MPM_EINVAL, "step_bwd: dyn_prev and sortbuf_prev go together");
+  if (ready < 0 || ready > 2) return set_err(MPM_EINVAL, "step_bwd: ready must be 0, 1 or 2");
   hipStream_t st = (hipStream_t)stream;
   Workspace w;
   carve(b, sizeof(T), (char*)ws, &w);
   const SortView<const int32_t> sv(b, sortbuf);
-  const int32_t* rank = sv.rank;
   const int bs = b->block;
+  T* const area[2] = {(T*)w.payload, (T*)w.payload2};
+  T* const pa = area[ready == 2 ? 1 : 0];
+  T* const pnext = area[ready == 2 ? 0 : 1];
   if (!ready) {
     TimedLaunch tl(MPM_K_PREP_ADJ, st);
     k_prep_adj<T><<<grid1(b->n_slots, bs), bs, 0, st>>>(b->inst, b->slot_inst, b->n_slots, dyn_in,
-                                                        bar_out, rank, (T*)w.payload, advect);
+                                                        bar_out, sv.rank, pa, advect);
   }
-  dispatch_adj<T>(tile_index(b), b, sv, w, grid, advect, st);
-  {
-  TimedLaunch tl(MPM_K_P2G_ADJ, st);
-  k_adj_gather<T><<<dim3((unsigned)(b->n_slots / kChunk)), kChunk, 0, st>>>(
-      b->inst, b->slot_inst, b->n_slots, sv.sorted, payload, (const T*)w.payload, grid,
-      (const T*)w.gadj, bar_out, bar_in, (T*)w.md, advect);
-  }
-  TimedLaunch tl(MPM_K_ADJ_VJP, st);
+  dispatch_adj<T>(tile_index(b), b, sv, w, pa, grid, advect, st);
+  TimedLaunch tl(MPM_K_ADJ_PARTICLE, st);
   per_model(b, 256, [&](auto M, const int32_t* ch, dim3 g) {
     constexpr int MODEL = decltype(M)::value;
     if (dyn_prev)
-      k_adj_vjp<T, MODEL, true><<<g, 256, 0, st>>>(
-          b->inst, b->slot_inst, b->n_slots, rank, dyn_in, Fg, sched, dyn_out, (const T*)w.md,
-          bar_out, bar_in, Fg_bar, theta_acc, dyn_prev, SortView<const int32_t>(b, sortbuf_prev).rank,
-          (T*)w.payload, advect, ch);
+      k_adj_particle<T, MODEL, true><<<g, 256, 0, st>>>(
+          b->inst, b->slot_inst, b->n_slots, sv.rank, payload, pa, grid, (const T*)w.gadj,
+          dyn_in, Fg, sched, dyn_out, bar_out, bar_in, Fg_bar, theta_acc, dyn_prev,
+          SortView<const int32_t>(b, sortbuf_prev).rank, pnext, advect, ch);
     else
-      k_adj_vjp<T, MODEL, false><<<g, 256, 0, st>>>(
-          b->inst, b->slot_inst, b->n_slots, rank, dyn_in, Fg, sched, dyn_out, (const T*)w.md,
-          bar_out, bar_in, Fg_bar, theta_acc, nullptr, nullptr, nullptr, advect, ch);
+      k_adj_particle<T, MODEL, false><<<g, 256, 0, st>>>(
+          b->inst, b->slot_inst, b->n_slots, sv.rank, payload, pa, grid, (const T*)w.gadj,
+          dyn_in, Fg, sched, dyn_out, bar_out, bar_in, Fg_bar, theta_acc, nullptr, nullptr,
+          nullptr, advect, ch);
   });
   return check_launch("step_bwd");
 }

@@ -342,13 +342,15 @@ class BatchPlan:
     def step_bwd_chain(self, dyn_in, G, sched_row, dyn_out, grid, sortbuf, payload, bar_out,
                        bar_in, G_bar, theta_acc, advect, ready, dyn_prev, sortbuf_prev,
                        stream=None):
-        """One reverse step of a chain: ``ready`` -- its adjoint payload comes
-        from the previous (later) chained step; ``dyn_prev``/``sortbuf_prev``
+        """One reverse step of a chain: ``ready`` -- 0 for the chain's first
+        call (its adjoint payload is computed), else 1 + (i % 2) for its i-th
+        call (the payload comes from the previous, later, chained step, in one
+        of the workspace's two payload areas); ``dyn_prev``/``sortbuf_prev``
         (or None) -- also prepare the previous step's."""
         f = getattr(_lib.lib(), f"mpm_step_bwd_chain_{suffix(dyn_in.dtype)}")
         rc = f(ctypes.byref(self.c), ptr(dyn_in), ptr(G), sched_row, ptr(dyn_out), ptr(grid),
                ptr(sortbuf), ptr(payload), ptr(bar_out), ptr(bar_in), ptr(G_bar), ptr(theta_acc),
-               int(bool(advect)), int(bool(ready)), ptr(dyn_prev), ptr(sortbuf_prev),
+               int(bool(advect)), int(ready), ptr(dyn_prev), ptr(sortbuf_prev),
                ptr(self.workspace(dyn_in.dtype)), stream_ptr(stream))
         check(rc, "step_bwd_chain")
 
@@ -546,7 +548,8 @@ class Trajectory:
             for k in range(L - 1, -1, -1):
                 self.plan.step_bwd_chain(self._src(s, k), self.G, self._a(t0 + k),
                                          self._dst(s, k), grid[k], sort[k], payload[k],
-                                         bar_out, bar_in, self.G_bar, th, self.advect, k < L - 1,
+                                         bar_out, bar_in, self.G_bar, th, self.advect,
+                                         0 if k == L - 1 else 1 + (L - 1 - k) % 2,
                                          self._src(s, k - 1) if k > 0 else None,
                                          sort[k - 1] if k > 0 else None, self.stream)
                 bar_out, bar_in = bar_in, bar_out

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     lib.mpm_abi_version.restype = ctypes.c_int32
-    assert lib.mpm_abi_version() == 5
+    assert lib.mpm_abi_version() == 6
 
 
 def test_ctypes_structs_match_c_layout(tmp_path):

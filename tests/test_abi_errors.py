@@ -135,6 +135,9 @@ def test_step_bwd_rejects_bad_buffers(suf):
     # dyn_prev without sortbuf_prev
     assert call(g, base, [1, 0, FAKE + 9000, None, FAKE + 8192, None]) == MPM_EINVAL
     assert "prev" in _msg()
+    # the payload area of a chained call is 0 (computed), 1 or 2
+    assert call(g, base, [1, 3, None, None, FAKE + 8192, None]) == MPM_EINVAL
+    assert "ready" in _msg()
 
 
 @pytest.mark.parametrize("suf", ["f32", "f64"])
