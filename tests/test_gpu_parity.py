@@ -153,8 +153,10 @@ def test_infer_growth_nudge_golden(golden):
 
 
 def test_rollout_checkpointed_equals_full_and_vjp():
-    """Checkpoint every 2 of 7 steps: identical forward and adjoint to one
-    segment, both against the oracle's stored-cache adjoint."""
+    """Checkpoint every 1, 2 or 3 of 7 steps: identical forward and adjoint to
+    one segment, all against the oracle's stored-cache adjoint.  The segment
+    lengths (7; 1; 2 2 2 1; 3 3 1) give reverse chains of odd and even length,
+    whose calls alternate between the workspace's two payload areas."""
     from morphompm_amd import integrate as G
     from morphompm_amd.config import SimConfig
     cfg = SimConfig()
@@ -166,14 +168,15 @@ def test_rollout_checkpointed_equals_full_and_vjp():
             rng.standard_normal((n, 3, 3)), rng.standard_normal((n, 3, 3)),
             rng.standard_normal((n, 3, 3)))
     outs = []
-    for K in (None, 2):
+    for K in (None, 1, 2, 3):
         st, h = G.rollout(G.iso_growth_state(pts, 1.2), m, cfg, 7, advect=True,
                           checkpoint_every=K)
         outs.append((st, G.rollout_vjp(h, *bars, advect=True)))
-    for f in ("x", "F", "C"):
-        assert np.array_equal(getattr(outs[0][0], f), getattr(outs[1][0], f))
-    for a, b in zip(outs[0][1], outs[1][1]):
-        assert np.array_equal(a, b)
+    for o in outs[1:]:
+        for f in ("x", "F", "C"):
+            assert np.array_equal(getattr(outs[0][0], f), getattr(o[0], f))
+        for a, b in zip(outs[0][1], o[1]):
+            assert np.array_equal(a, b)
     ost, oc = RV.rollout(RV.iso_growth_state(pts, 1.2), RL.Hencky(cfg.material), cfg, 7,
                          advect=True)
     ob = RV.rollout_vjp(oc, *bars, advect=True)
