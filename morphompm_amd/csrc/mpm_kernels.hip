@@ -228,14 +228,17 @@ constexpr int RX = 0, RV = 3, RC = 6, RF = 15;
 // ---------------------------------------------------------------------------
 
 // Unit (tile) of in-grid bin (bx, by, bz) and the bin's place in the tile.
+// Every built tile shape has power-of-two sides and 0 <= shift < tile, so the
+// shifted bin is non-negative and / and % by a side are a shift and a mask.
 __device__ __forceinline__ void bin_unit(const mpm_instance& I, int bx, int by, int bz, int TX,
                                          int TY, int TZ, int64_t& unit, int& local) {
   bx += I.shift[0];
   by += I.shift[1];
   bz += I.shift[2];
-  const int itx = bx / TX, ity = by / TY, itz = bz / TZ;
+  const int itx = bx >> __builtin_ctz(TX), ity = by >> __builtin_ctz(TY),
+            itz = bz >> __builtin_ctz(TZ);
   unit = I.t_off + ((int64_t)itx * I.ty + ity) * I.tz + itz;
-  local = ((bx - itx * TX) * TY + (by - ity * TY)) * TZ + (bz - itz * TZ);
+  local = ((bx & (TX - 1)) * TY + (by & (TY - 1))) * TZ + (bz & (TZ - 1));
 }
 
 __device__ __forceinline__ int64_t dead_unit(const mpm_instance& I) {
@@ -742,6 +745,22 @@ __device__ __forceinline__ void gather_tiles(const Inst<T>& I, const int32_t* un
   }
 }
 
+// Node (i, j, k) of global node g in its instance's grid; false for the
+// padding nodes past the grid.  An instance grid holds at most 1022^3 < 2^31
+// nodes, so 32-bit division is exact (the 64-bit one costs several times more).
+template <typename T>
+__device__ __forceinline__ bool node_of(const Inst<T>& I, int64_t g, int& i, int& j, int& k) {
+  const int64_t local = g - I.g_off;
+  const int plane = I.ny * I.nz;
+  if (local >= (int64_t)I.nx * plane) return false;
+  const int l = (int)local;
+  i = l / plane;
+  const int r = l - i * plane;
+  j = r / I.nz;
+  k = r - j * I.nz;
+  return true;
+}
+
 // Grid update (transfer.py:56-59): v = (1 - damping) mom / m where m > 1e-12.
 // grid record per node: (m, vx, vy, vz).
 template <typename T, int TX, int TY, int TZ>
@@ -751,10 +770,8 @@ __global__ __launch_bounds__(256) void k_grid_fwd(const mpm_instance* inst, cons
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_nodes) return;
   const Inst<T> I = load_inst<T>(inst, block_inst(node_inst));
-  const int64_t local = g - I.g_off;
-  if (local >= (int64_t)I.nx * I.ny * I.nz) return;
-  const int k = (int)(local % I.nz), j = (int)((local / I.nz) % I.ny),
-            i = (int)(local / ((int64_t)I.nz * I.ny));
+  int i, j, k;
+  if (!node_of(I, g, i, j, k)) return;
   T acc[4];
   gather_tiles<T, TX, TY, TZ>(I, unit_rank, partial, i, j, k, acc);
   const T m = I.pmass * acc[0];
@@ -1039,10 +1056,8 @@ __global__ __launch_bounds__(256) void k_grid_adj(const mpm_instance* inst, cons
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_nodes) return;
   const Inst<T> I = load_inst<T>(inst, block_inst(node_inst));
-  const int64_t local = g - I.g_off;
-  if (local >= (int64_t)I.nx * I.ny * I.nz) return;
-  const int k = (int)(local % I.nz), j = (int)((local / I.nz) % I.ny),
-            i = (int)(local / ((int64_t)I.nz * I.ny));
+  int i, j, k;
+  if (!node_of(I, g, i, j, k)) return;
   T acc[4];
   gather_tiles<T, TX, TY, TZ>(I, unit_rank, partial, i, j, k, acc);
   const V4<T> gr = ld4(grid + 4 * g);
