@@ -801,8 +801,10 @@ struct NodeBox {
   bool staged;
 };
 
-// Block-wide bounding box of the valid particles' bases (blockDim 256).
-template <int CAP>
+// Block-wide bounding box of the valid particles' bases.  NT: the block size
+// when fixed at compile time, else 0 (blockDim 64, 128 or 256); red holds 6
+// ints per wave.
+template <int CAP, int NT>
 __device__ __forceinline__ NodeBox chunk_box(bool valid, const int base[3], int* red) {
   int v[6];
   for (int c = 0; c < 3; ++c) {
@@ -820,9 +822,14 @@ __device__ __forceinline__ NodeBox chunk_box(bool valid, const int base[3], int*
   __syncthreads();
   NodeBox b;
   int hi[3];
+  const int nw = (NT ? NT : (int)blockDim.x) >> 6;
   for (int c = 0; c < 3; ++c) {
-    b.lo[c] = min(min(red[c], red[6 + c]), min(red[12 + c], red[18 + c]));
-    hi[c] = max(max(red[3 + c], red[9 + c]), max(red[15 + c], red[21 + c]));
+    b.lo[c] = red[c];
+    hi[c] = red[3 + c];
+    for (int w = 1; w < nw; ++w) {
+      b.lo[c] = min(b.lo[c], red[6 * w + c]);
+      hi[c] = max(hi[c], red[6 * w + 3 + c]);
+    }
   }
   const bool any = hi[0] >= b.lo[0];
   for (int c = 0; c < 3; ++c) b.d[c] = any ? hi[c] - b.lo[c] + 3 : 0;
@@ -838,13 +845,14 @@ __device__ __forceinline__ int small_div(int n, float inv_d) {
   return (int)(((float)n + 0.5f) * inv_d);
 }
 
-// lds[n] = record of box node n (x-major), zero outside the grid.
-template <typename T>
+// lds[n] = record of box node n (x-major), zero outside the grid; NT as in
+// chunk_box.
+template <int NT, typename T>
 __device__ __forceinline__ void stage_box(const Inst<T>& I, const NodeBox& b, const T* rec,
                                           V4<T>* lds) {
   const int plane = b.d[1] * b.d[2];  // <= b.vol <= the box capacity
   const float inv_plane = 1.0f / (float)plane, inv_d2 = 1.0f / (float)b.d[2];
-  for (int n = threadIdx.x; n < b.vol; n += 256) {
+  for (int n = threadIdx.x; n < b.vol; n += (NT ? NT : (int)blockDim.x)) {
     const int i = small_div(n, inv_plane), r = n - i * plane;
     const int j = small_div(r, inv_d2), k = r - j * b.d[2];
     const int gx = b.lo[0] + i, gy = b.lo[1] + j, gz = b.lo[2] + k;
@@ -917,7 +925,7 @@ __device__ __forceinline__ GridNodes<T> grid_nodes(const T* rec, const Inst<T>& 
 
 // ---------------------------------------------------------------------------
 // G2P (transfer.py:61-77), one thread per particle slot (coalesced state
-// reads and writes); stencil nodes come from the grid through the caches.
+// reads and writes); stencil nodes come from the block's box staged in LDS.
 // NEXT: also writes the next step's payload, sort key, bin count and unit
 // flag from the new state (what k_prep_fwd would compute from dyn_out, bit for
 // bit), with the next step's growth factor sched_next.
@@ -930,20 +938,33 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
                                              T* payload_next, int32_t* key, int32_t* count,
                                              int32_t* unit_flag, int out,
                                              const int32_t* chunks) {
+  constexpr int CAP = BoxCap<T>::value;
+  __shared__ V4<T> lg[CAP];  // the block's stencil box of the grid
+  __shared__ int red[24];
   const int64_t p0 = block_first_slot(chunks), p = p0 + threadIdx.x;
-  if (p0 >= n_slots) return;
+  if (p0 >= n_slots) return;  // whole blocks
   const int ii = __builtin_amdgcn_readfirstlane(slot_inst[p0 / kChunk]);
   const Inst<T> I = load_inst<T, MODEL>(inst, ii);
   const int64_t local = p - I.p_off;
-  if (!NEXT && local >= I.n) return;
-  // (with NEXT, whole waves only leave early: count_in_bin needs every lane)
-  int32_t next_key = -1;
-  if (local < I.n) {
-    T x[3], fx[3];
-    int base[3];
+  const bool live = local < I.n;
+  T x[3] = {T(0), T(0), T(0)}, fx[3] = {T(0), T(0), T(0)};
+  int base[3] = {0, 0, 0};
+  bool valid = false;
+  if (live) {
     ld3_nt(dyn_in, n_slots, p, RX, x);
     stencil_base(x, I, base, fx);
-    const bool valid = base_in_grid(base, I.nx, I.ny, I.nz);
+    valid = base_in_grid(base, I.nx, I.ny, I.nz);
+  }
+  // Particles of consecutive slots are spatially close, so the box spanned by
+  // the block's stencils is small; it is staged once in LDS and the 27-node
+  // gathers read it there (a box beyond the LDS capacity reads the grid).
+  const NodeBox bx = chunk_box<CAP, 0>(valid, base, red);
+  if (bx.staged) stage_box<0>(I, bx, grid, lg);
+  __syncthreads();
+  if (!NEXT && !live) return;
+  // (with NEXT, every lane stays: count_in_bin needs every lane)
+  int32_t next_key = -1;
+  if (live) {
     // Separable weights: with d = (o - fx) dx,
     //   nv = sum w v,   nC = k4 dx (sum w v o^T - nv fx^T),
     // and each z-row (i, j) of 3 nodes is summed first.
@@ -982,7 +1003,12 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
         nC(c, 2) = s * (Sk[c] - nv[c] * fx[2]);
       }
     };
-    if (valid) gather(grid_nodes(grid, I, base));
+    if (valid) {
+      if (bx.staged)
+        gather(box_nodes(lg, bx, base));
+      else
+        gather(grid_nodes(grid, I, base));
+    }
     const M3<T> F = ld9_nt(dyn_in, n_slots, p, RF);
     // F_new = (I + dt C) F, formed as F + dt (C F): I + dt C would round away
     // the low bits of the small increment the same way for every particle.
@@ -1127,10 +1153,10 @@ __global__ __launch_bounds__(256) void k_adj_particle(
   const bool valid = live && cell != kNoCell;
   const int base[3] = {(int)(cell & 1023u) - 2, (int)((cell >> 10) & 1023u) - 2,
                        (int)(cell >> 20) - 2};
-  const NodeBox bx = chunk_box<CAP>(valid, base, red);
+  const NodeBox bx = chunk_box<CAP, 256>(valid, base, red);
   if (bx.staged) {
-    stage_box(I, bx, gadj, lg);
-    if (advect) stage_box(I, bx, grid, lv);
+    stage_box<256>(I, bx, gadj, lg);
+    if (advect) stage_box<256>(I, bx, grid, lv);
   }
   __syncthreads();
   T th[2] = {T(0), T(0)};
