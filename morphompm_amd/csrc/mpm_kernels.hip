@@ -623,8 +623,8 @@ __global__ __launch_bounds__(256) void k_cell_keys(const mpm_instance* inst,
 // P2G as a cell-binned segmented reduction into an LDS node tile.
 // Tile = TX*TY*TZ bins (contiguous keys), one thread each; node tile
 // (TX+2)(TY+2)(TZ+2).  One block per occupied unit of this step (blocks past
-// n_active and dead units return at once).  partial: [active unit][4][TN] --
-// row 0 sum of weights (MASS), rows 1-3 sum of w (Q + A' o).
+// n_active and dead units return at once).  partial: [active unit][TN] node
+// records (sum of weights (MASS), sum of w (Q + A' o)), one quad per node.
 // ---------------------------------------------------------------------------
 // INDIRECT: payload records in slot order, read through sorted[j]; else in
 // sorted order.
@@ -639,7 +639,7 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
                                                         const T* payload, T* partial) {
   constexpr int NY = TY + 2, NZ = TZ + 2, TN = (TX + 2) * NY * NZ;
   constexpr int NT = TX * TY * TZ;
-  __shared__ T lds[4 * TN];
+  __shared__ V4<T> lds[TN];
   const int64_t a = blockIdx.x;
   if (a >= *n_active) return;
   const int32_t u = active_unit[a];
@@ -649,7 +649,7 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
   const int tid = threadIdx.x;
   const int lz = tid % TZ, ly = (tid / TZ) % TY, lx = tid / (TY * TZ);
 
-  for (int i = tid; i < 4 * TN; i += NT) lds[i] = T(0);
+  for (int i = tid; i < TN; i += NT) lds[i] = V4<T>{T(0), T(0), T(0), T(0)};
   // INDIRECT: the tile's particles are the contiguous sorted range [r0, r1);
   // their slot indices are staged in LDS with coalesced loads, so the record
   // loads do not wait on a dependent index load each.
@@ -703,14 +703,19 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
   for (int o = 0; o < 27; ++o) {
     __syncthreads();
     const int nl = ((lx + o / 9) * NY + (ly + (o / 3) % 3)) * NZ + (lz + o % 3);
-    if (MASS) lds[nl] += aw[o];
-    lds[TN + nl] += am[o][0];
-    lds[2 * TN + nl] += am[o][1];
-    lds[3 * TN + nl] += am[o][2];
+    V4<T> t = lds[nl];
+    if (MASS) t.x += aw[o];
+    t.y += am[o][0];
+    t.z += am[o][1];
+    t.w += am[o][2];
+    lds[nl] = t;
   }
   __syncthreads();
   T* out = partial + a * (int64_t)(4 * TN);
-  for (int i = tid; i < 4 * TN; i += NT) out[i] = lds[i];
+  for (int i = tid; i < TN; i += NT) {
+    const V4<T> t = lds[i];
+    st4(out + 4 * i, t.x, t.y, t.z, t.w);
+  }
 }
 
 // Sum of the occupied tile partials covering node (i, j, k), tiles in
@@ -735,11 +740,11 @@ __device__ __forceinline__ void gather_tiles(const Inst<T>& I, const int32_t* un
         const int64_t lt = ((int64_t)ix * I.ty + iy) * I.tz + iz;
         const int32_t a = unit_rank[I.t_off + lt];
         if (a < 0) continue;
-        const T* src = partial + a * (int64_t)(4 * TN) + (li * NY + lj) * NZ + lk;
-        acc[0] += src[0];
-        acc[1] += src[TN];
-        acc[2] += src[2 * TN];
-        acc[3] += src[3 * TN];
+        const V4<T> v = ld4(partial + a * (int64_t)(4 * TN) + 4 * ((li * NY + lj) * NZ + lk));
+        acc[0] += v.x;
+        acc[1] += v.y;
+        acc[2] += v.z;
+        acc[3] += v.w;
       }
     }
   }
