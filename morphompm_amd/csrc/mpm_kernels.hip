@@ -718,36 +718,58 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
   }
 }
 
+// The tiles of side TS along one axis that cover node n (s = n + 2 + shift):
+// tile ix covers nodes [ix TS - shift - 2, ix TS - shift + TS - 1], so node n
+// lies in tile hi = s / TS at place lo = s - hi TS (0 <= lo < TS), and also in
+// tile hi - 1 at place lo + TS when lo <= 1: at most two, in ascending order.
+template <int TS>
+__device__ __forceinline__ void tile_cover(int s, int t[2], int l[2], bool v[2]) {
+  const int hi = s / TS, lo = s - hi * TS;
+  t[0] = hi - 1;
+  l[0] = lo + TS;
+  v[0] = lo <= 1 && hi >= 1;
+  t[1] = hi;
+  l[1] = lo;
+  v[1] = true;
+}
+
 // Sum of the occupied tile partials covering node (i, j, k), tiles in
-// ascending order.
+// ascending order.  The (at most 8) unit ranks are loaded together, then the
+// partial records, so the dependent loads of different tiles overlap.
 template <typename T, int TX, int TY, int TZ>
 __device__ __forceinline__ void gather_tiles(const Inst<T>& I, const int32_t* unit_rank,
                                              const T* partial, int i, int j, int k, T acc[4]) {
   constexpr int NY = TY + 2, NZ = TZ + 2, TN = (TX + 2) * NY * NZ;
-  acc[0] = acc[1] = acc[2] = acc[3] = T(0);
-  // tile ix covers nodes [ix TX - sx - 2, ix TX - sx + TX - 1]
-  const int xhi = min((i + 2 + I.sx) / TX, I.tx - 1), yhi = min((j + 2 + I.sy) / TY, I.ty - 1),
-            zhi = min((k + 2 + I.sz) / TZ, I.tz - 1);
-  for (int ix = max(0, xhi - 2); ix <= xhi; ++ix) {
-    const int li = i - (ix * TX - I.sx - 2);
-    if (li < 0 || li >= TX + 2) continue;
-    for (int iy = max(0, yhi - 2); iy <= yhi; ++iy) {
-      const int lj = j - (iy * TY - I.sy - 2);
-      if (lj < 0 || lj >= TY + 2) continue;
-      for (int iz = max(0, zhi - 2); iz <= zhi; ++iz) {
-        const int lk = k - (iz * TZ - I.sz - 2);
-        if (lk < 0 || lk >= TZ + 2) continue;
-        const int64_t lt = ((int64_t)ix * I.ty + iy) * I.tz + iz;
-        const int32_t a = unit_rank[I.t_off + lt];
-        if (a < 0) continue;
-        const V4<T> v = ld4(partial + a * (int64_t)(4 * TN) + 4 * ((li * NY + lj) * NZ + lk));
-        acc[0] += v.x;
-        acc[1] += v.y;
-        acc[2] += v.z;
-        acc[3] += v.w;
-      }
-    }
+  int ct[3][2], cl[3][2];
+  bool cv[3][2];
+  tile_cover<TX>(i + 2 + I.sx, ct[0], cl[0], cv[0]);
+  tile_cover<TY>(j + 2 + I.sy, ct[1], cl[1], cv[1]);
+  tile_cover<TZ>(k + 2 + I.sz, ct[2], cl[2], cv[2]);
+  int32_t ar[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int x = c >> 2, y = (c >> 1) & 1, z = c & 1;
+    ar[c] = -1;
+    if (cv[0][x] && cv[1][y] && cv[2][z])
+      ar[c] = unit_rank[I.t_off + ((int64_t)ct[0][x] * I.ty + ct[1][y]) * I.tz + ct[2][z]];
   }
+  V4<T> pv[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int x = c >> 2, y = (c >> 1) & 1, z = c & 1;
+    pv[c] = V4<T>{T(0), T(0), T(0), T(0)};
+    if (ar[c] >= 0)
+      pv[c] = ld4(partial + ar[c] * (int64_t)(4 * TN) + 4 * ((cl[0][x] * NY + cl[1][y]) * NZ + cl[2][z]));
+  }
+  acc[0] = acc[1] = acc[2] = acc[3] = T(0);
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    if (ar[c] >= 0) {
+      acc[0] += pv[c].x;
+      acc[1] += pv[c].y;
+      acc[2] += pv[c].z;
+      acc[3] += pv[c].w;
+    }
 }
 
 // Node (i, j, k) of global node g in its instance's grid; false for the
