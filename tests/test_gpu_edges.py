@@ -152,6 +152,34 @@ def test_largest_axis_grid_fp64():
         assert _rel(a, b) < 1e-10, f
 
 
+@pytest.mark.parametrize("order", ["random", "cells"])
+def test_staged_and_direct_stencil_boxes_fp64(order):
+    """G2P and the per-particle reverse kernel read the stencil nodes of a
+    block of 256 consecutive slots from the block's box of nodes staged in LDS
+    when it fits (640 nodes in FP64), else straight from the grid.  2048
+    particles in a 7-cell cube: in random slot order every block's box is the
+    whole cube (the direct path); sorted by cell, every box holds a few hundred
+    nodes (the staged path).  Both match the FP64 oracle's step and VJP."""
+    dx = 1.0 / 24
+    rng = np.random.default_rng(5)
+    n = 2048
+    pts = rng.uniform(0.3, 0.6, (n, 3))
+    base = np.floor(pts / dx - 0.5).astype(np.int64)
+    if order == "cells":
+        o = np.lexsort((base[:, 2], base[:, 1], base[:, 0]))
+        pts, base = pts[o], base[o]
+    vol = [int(np.prod(b.max(0) - b.min(0) + 3)) for b in base.reshape(-1, 256, 3)]
+    assert (min(vol) > 640) if order == "random" else (max(vol) <= 640), vol
+    arrs = (pts, 0.3 * rng.standard_normal((n, 3)), 0.2 * rng.standard_normal((n, 3, 3)),
+            I3 + 0.05 * rng.standard_normal((n, 3, 3)), 1.05 * np.repeat(I3[None], n, 0))
+    cfgs = _cfgs(N=24, dx=dx, dt=1e-4, sp=dx / 2)
+    ns, rs, gv, rv = _step_both(arrs, "hencky", 1, cfgs)
+    for f in FIELDS:
+        assert _rel(getattr(ns, f), getattr(rs, f)) < 1e-12, f
+    for f, a, b in zip(FIELDS, gv, rv[:5]):
+        assert _rel(a, b) < 1e-10, f
+
+
 # ------------------------------------------- empty and tiny batch instances
 def _scenes_with_empty_and_single():
     from morphompm_amd import constitutive as K
