@@ -367,19 +367,22 @@ __global__ void k_scan_blocks(int32_t* count, const int32_t* active_unit, int32_
   const int64_t n = (int64_t)__builtin_amdgcn_readfirstlane(*n_active) * TB;
   const int64_t i0 = ((int64_t)blockIdx.x * kScanBlock + threadIdx.x) * kScanPer;
   if ((int64_t)blockIdx.x * kScanBlock * kScanPer >= n) return;
-  int32_t v[kScanPer];
-  int32_t tot = 0;
-  for (int k = 0; k < kScanPer; ++k) {
-    v[k] = 0;
-    const int64_t i = i0 + k;
-    if (i < n) {
-      const int64_t c = (int64_t)active_unit[i / TB] * TB + i % TB;
-      v[k] = count[c];
-      count[c] = 0;
-      cursor[i] = 0;
-    }
-    tot += v[k];
+  static_assert(kScanPer == 4, "one 16-byte quad of bins per thread");
+  int32_t v[kScanPer] = {0, 0, 0, 0};
+  if (i0 < n) {
+    // TB (a product of power-of-two tile sides) is a multiple of 4 and so is
+    // n, so the thread's 4 bins are places l0 .. l0 + 3 of one unit
+    const int32_t a = (int32_t)i0 / TB, l0 = (int32_t)i0 - a * TB;
+    int32_t* cp = count + (int64_t)active_unit[a] * TB + l0;
+    const int4 c4 = *reinterpret_cast<const int4*>(cp);
+    v[0] = c4.x;
+    v[1] = c4.y;
+    v[2] = c4.z;
+    v[3] = c4.w;
+    *reinterpret_cast<int4*>(cp) = make_int4(0, 0, 0, 0);
+    *reinterpret_cast<int4*>(cursor + i0) = make_int4(0, 0, 0, 0);
   }
+  const int32_t tot = v[0] + v[1] + v[2] + v[3];
   sh[threadIdx.x] = tot;
   __syncthreads();
   for (int off = 1; off < kScanBlock; off <<= 1) {
@@ -428,14 +431,16 @@ __global__ __launch_bounds__(256) void k_place(const int32_t* unit_rank, int TB,
   const int64_t w0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 * G);
   if (w0 >= n_slots) return;  // whole waves (n_slots is a multiple of 256)
   int32_t raw[G], k[G], s0[G], my_head[G], base[G];
+  const int tb_shift = __builtin_ctz(TB);  // TB is a power of two
 #pragma unroll
   for (int i = 0; i < G; ++i) raw[i] = key[w0 + i * 64 + lane];
 #pragma unroll
   for (int i = 0; i < G; ++i) {
     k[i] = -1;
     if (raw[i] >= 0) {
-      const int32_t ur = raw[i] / TB < n_units ? unit_rank[raw[i] / TB] : -1;
-      if (ur >= 0) k[i] = ur * TB + raw[i] % TB;
+      const int32_t u = raw[i] >> tb_shift;
+      const int32_t ur = u < n_units ? unit_rank[u] : -1;
+      if (ur >= 0) k[i] = ur * TB + (raw[i] & (TB - 1));
     }
     s0[i] = k[i] >= 0 ? start[k[i]] : 0;
   }
