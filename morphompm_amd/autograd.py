@@ -50,10 +50,23 @@ def _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched, keep_segmen
     plan = BatchPlan([spec], tile=tile or DEFAULT_TILE, block=block)
     traj = Trajectory(plan, x0.dtype, n_steps, advect, checkpoint_every, sched,
                       keep_segments=keep_segments)
-    return plan, traj
+    return plan, traj, _cell_perm(plan, x0)
 
 
-def _pack(plan, x0, v0, C0, F0, Fg):
+def _cell_perm(plan, x0):
+    """Slot order of the particles: by bin of the rest positions (the caller's
+    order comes back out of ``_unpack``).  Consecutive slots then hold nearby
+    particles, so the stencil boxes that G2P and the reverse kernel stage in
+    LDS per block of slots stay small whatever order the caller uses."""
+    perm = torch.as_tensor(plan.cell_order([x0.detach()])[0], device=x0.device)  # sorted on x0's device
+    inv = torch.empty_like(perm)
+    inv[perm] = torch.arange(perm.numel(), device=x0.device)
+    return perm, inv
+
+
+def _pack(plan, order, x0, v0, C0, F0, Fg):
+    perm = order[0]
+    x0, v0, C0, F0, Fg = (t[perm] for t in (x0, v0, C0, F0, Fg))
     n = x0.shape[0]
     sl = plan.sl(0)
     pad = plan.n_slots
@@ -64,11 +77,12 @@ def _pack(plan, x0, v0, C0, F0, Fg):
     return dyn0, G
 
 
-def _unpack(plan, fin, n, dtype):
+def _unpack(plan, order, fin, n, dtype):
+    inv = order[1]
     f = fin[:, plan.sl(0)]
     x = (f[ROW_X:ROW_X + 3].T.to(torch.float64) + plan.origin_offset(0)).to(dtype)
-    return (x, f[ROW_V:ROW_V + 3].T, f[ROW_C:ROW_C + 9].T.reshape(n, 3, 3),
-            f[ROW_F:ROW_F + 9].T.reshape(n, 3, 3))
+    return (x[inv], f[ROW_V:ROW_V + 3].T[inv], f[ROW_C:ROW_C + 9].T[inv].reshape(n, 3, 3),
+            f[ROW_F:ROW_F + 9].T[inv].reshape(n, 3, 3))
 
 
 def _check_hb(model, hb_params):
@@ -84,11 +98,11 @@ def rollout(x0, v0, C0, F0, Fg, model, cfg, n_steps, advect=False, *, checkpoint
     and particle block size (None: the default tile; "auto": the autotuner,
     cached per shape class); ``keep_segments``: see ``engine.Trajectory``."""
     _check_hb(model, hb_params)
-    plan, traj = _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched,
-                        keep_segments, tile, block, Fg)
-    dyn0, G = _pack(plan, x0, v0, C0, F0, Fg)
+    plan, traj, order = _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched,
+                               keep_segments, tile, block, Fg)
+    dyn0, G = _pack(plan, order, x0, v0, C0, F0, Fg)
     fin = _Rollout.apply(dyn0, G, hb_params, traj, plan, plan.specs[0].prm)
-    return _unpack(plan, fin, x0.shape[0], x0.dtype)
+    return _unpack(plan, order, fin, x0.shape[0], x0.dtype)
 
 
 class _GraphRollout(torch.autograd.Function):
@@ -123,8 +137,9 @@ class GraphRollout:
         if hb and getattr(model, "kind", None) != "hb":
             raise ValueError("hb applies to HerschelBulkley models only")
         self.model, self.n, self.dtype, self.hb = model, x0.shape[0], x0.dtype, bool(hb)
-        self.plan, self.traj = _setup(x0, model, cfg, n_steps, advect, checkpoint_every, sched,
-                                      keep_segments, tile, block, Fg)
+        self.plan, self.traj, self.order = _setup(x0, model, cfg, n_steps, advect,
+                                                  checkpoint_every, sched, keep_segments, tile,
+                                                  block, Fg)
         p = self.plan
         self.prm_base = list(p.specs[0].prm)
         self.dyn_in = p.new_dyn(self.dtype)
@@ -178,6 +193,6 @@ class GraphRollout:
             raise ValueError("hb_params must be given exactly when the runner was built with hb=True")
         if x0.shape[0] != self.n or x0.dtype != self.dtype:
             raise ValueError(f"runner built for {self.n} particles of {self.dtype}")
-        dyn0, G = _pack(self.plan, x0, v0, C0, F0, Fg)
+        dyn0, G = _pack(self.plan, self.order, x0, v0, C0, F0, Fg)
         fin = _GraphRollout.apply(dyn0, G, hb_params, self)
-        return _unpack(self.plan, fin, self.n, self.dtype)
+        return _unpack(self.plan, self.order, fin, self.n, self.dtype)

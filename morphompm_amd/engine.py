@@ -97,6 +97,31 @@ def _ceil(a, b):
     return -(-a // b)
 
 
+def bin_keys(x, dx, shape, tile, shift):
+    """Unit-level bin key (tile id * cells per tile + place in the tile) of
+    positions x (n, 3), numpy or torch (int64 result of the same kind), for a
+    grid of ``shape`` nodes, the P2G tile shape and the instance's tile shift.
+    Positions off the grid are clamped to its border bins."""
+    if isinstance(x, torch.Tensor):
+        arr = lambda v: torch.tensor(v, dtype=torch.int64, device=x.device)
+        b = torch.floor(x * (1.0 / dx) - 0.5).to(torch.int64) + 2
+        clip = lambda v, lo, hi: torch.minimum(torch.maximum(v, lo), hi)
+    else:
+        arr = lambda v: np.array(v, dtype=np.int64)
+        b = np.floor(x * (1.0 / dx) - 0.5).astype(np.int64) + 2
+        clip = np.clip
+    dims = arr(list(shape)) + 2
+    b = clip(b, arr([0, 0, 0]), dims - 1)
+    T, sh = arr(list(tile)), arr(list(shift))
+    b = b + sh
+    t = b // T
+    nt = -(-(dims + sh) // T)
+    tile_id = (t[:, 0] * nt[1] + t[:, 1]) * nt[2] + t[:, 2]
+    l = b - t * T
+    local = (l[:, 0] * T[1] + l[:, 1]) * T[2] + l[:, 2]
+    return tile_id * int(np.prod(tile)) + local
+
+
 class BatchPlan:
     def __init__(self, specs, tile=DEFAULT_TILE, block=256, device="cuda"):
         self.specs = list(specs)
@@ -225,23 +250,20 @@ class BatchPlan:
         return slice(self.p_off[i], self.p_off[i] + self.specs[i].n)
 
     def cell_order(self, pts_list):
+        """Per instance, the stable order of its particles by bin key (tile,
+        then place in the tile).  A CUDA tensor of positions gets a device
+        tensor back, sorted on the device; anything else a numpy array."""
         perms = []
         for i, pts in enumerate(pts_list):
             s = self.specs[i]
-            x = np.asarray(pts.detach().cpu() if isinstance(pts, torch.Tensor) else pts,
-                           dtype=np.float64)
-            b = np.floor(x * (1.0 / s.dx) - 0.5).astype(np.int64) + 2
-            dims = np.array(s.shape) + 2
-            b = np.clip(b, 0, dims - 1)
-            T = np.array(self.tile)
-            sh = np.array([self._insts[i].shift[a] for a in range(3)])
-            b = b + sh
-            t = b // T
-            nt = -(-(dims + sh) // T)
-            tile_id = (t[:, 0] * nt[1] + t[:, 1]) * nt[2] + t[:, 2]
-            l = b - t * T
-            local = (l[:, 0] * T[1] + l[:, 1]) * T[2] + l[:, 2]
-            perms.append(np.argsort(tile_id * int(T.prod()) + local, kind="stable"))
+            sh = [self._insts[i].shift[a] for a in range(3)]
+            if isinstance(pts, torch.Tensor) and pts.is_cuda:
+                key = bin_keys(pts.detach().to(torch.float64), s.dx, s.shape, self.tile, sh)
+                perms.append(torch.argsort(key, stable=True))
+            else:
+                x = np.asarray(pts.detach().cpu() if isinstance(pts, torch.Tensor) else pts,
+                               dtype=np.float64)
+                perms.append(np.argsort(bin_keys(x, s.dx, s.shape, self.tile, sh), kind="stable"))
         return perms
 
     def _cols(self, a, i, n, k, dtype, perm):
