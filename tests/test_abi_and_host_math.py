@@ -187,3 +187,23 @@ def test_host_vjp_fd(name):
             fd = (Lp - Lm) / (2 * eps)
             an = np.sum(Fb[0] * dF) + np.sum(Gb[0] * dG) + np.sum(Cb[0] * dC)
             assert abs(an - fd) <= 1e-5 * (abs(fd) + 1e-12)
+
+
+def test_bin_keys_numpy_and_torch_agree():
+    """The slot order of the batched and single-scene paths sorts particles by
+    bin key; the key is computed with numpy on the host or with torch on the
+    device, and both give the same integers (off-grid positions clamp to the
+    border bins)."""
+    import torch
+    from morphompm_amd.engine import bin_keys
+    rng = np.random.default_rng(0)
+    for shape, tile, shift in [((20, 30, 17), (4, 4, 8), (1, 3, 5)), ((8, 8, 8), (2, 4, 8), (0, 0, 0)),
+                               ((33, 9, 70), (8, 8, 4), (7, 2, 3))]:
+        dx = 0.05
+        x = rng.uniform(-0.2, 1.2, (5000, 3)) * np.array(shape) * dx
+        a = bin_keys(x, dx, shape, tile, shift)
+        b = bin_keys(torch.tensor(x), dx, shape, tile, shift).numpy()
+        assert np.array_equal(a, b)
+        T = np.array(tile)
+        nt = -(-(np.array(shape) + 2 + np.array(shift)) // T)
+        assert a.min() >= 0 and a.max() < int(np.prod(nt)) * int(np.prod(T))
