@@ -1,19 +1,25 @@
 """Forward + adjoint particle-steps/s of the batched growth inversion on one MI355X.
 
-Workload (default): 256 independent free-swell / bilayer instances, particle
-counts log-uniform in [1k, 32k] (seeded), growth rates U(0.02, 0.2), 200 steps
-each, adjoint with checkpoints every 20 steps, FP32.  One bench step = one
-Gauss-Newton iteration of the batched inversion: the forward trajectory and the
-checkpointed adjoint (recompute included) of every instance, captured as HIP
-graphs and replayed, plus the per-instance update on the host.
+Workload (default): 256 independent instances, particle counts log-uniform in
+[1k, 32k] (seeded), growth rates U(0.02, 0.2): neo-Hookean free-swell cubes
+(even instances) and Hencky bilayer strips (odd instances), 200 steps each,
+adjoint with checkpoints every 20 steps, FP32.  One bench step = one
+Gauss-Newton iteration of the batched inversion: growth tensors from the
+rates, the forward trajectory, the loss, the checkpointed adjoint (recompute
+included) and the gradient of every instance, and the per-instance update --
+all on the device, captured once as one HIP graph and replayed (the timed
+region has no host round trip).  After the timed iterations the inversion
+continues to 50 iterations and the bench reports how well each instance's rate
+is recovered.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
 Prints one JSON line.  Besides the metric it reports the dominant kernel's
-HBM roofline fraction (timed with HIP events inside the graph replays of the
-timed region), the plain single-instance GPU rate (same kernels, one instance
-at a time, no graph), FP32 error vs an FP64 GPU run, and the FP64 numpy
-oracle timed on the host cores.
+HBM roofline fraction against the bytes the reference data model moves (timed
+with HIP events over one eager iteration after the timed region), the plain
+single-instance GPU rate (same kernels, one instance at a time, no graph),
+FP32 error vs an FP64 GPU run, and the FP64 numpy oracle timed on the host
+cores.
 """
 import argparse
 import json
@@ -71,6 +77,31 @@ def adj_particle_bytes_per_particle(n_steps, K):
     return total / n_steps
 
 
+# Bytes the reference data model (full 3x3 Fg, python/morphompm/state.py:10-16)
+# moves per particle-step in FP32, independent of this implementation:
+#   forward particles: read x, v, C, F, Fg (33 words), write x, v, C, F (24)  228 B
+#   forward grid, per active node: P2G write m + mom (4), update read 4 /
+#     write 3, G2P read 3                                                   56 B
+#   reverse particles: read the input state (33), the next step's C (9), the
+#     output cotangents x, v, C, F (24) and Fg_bar (9); write the input
+#     cotangents (24) and Fg_bar (9)                                        432 B
+#   reverse grid, per active node: v + m (4), v_bar write / read (3 + 3),
+#     mom_bar + m_bar write / read (4 + 4)                                  72 B
+# The per-particle reverse kernel (P2G-adjoint gather, constitutive VJP and
+# the previous step's G2P-adjoint payload) does all of the reverse particle
+# work, so its roofline is priced at 432 B per particle.
+REF_FWD_PARTICLE, REF_FWD_NODE = 228, 56
+REF_ADJ_PARTICLE, REF_ADJ_NODE = 432, 72
+
+
+def reference_bytes_per_particle_step(r):
+    """Whole forward + adjoint step, r = active nodes per particle (checkpoint
+    recompute not counted)."""
+    return REF_FWD_PARTICLE + REF_ADJ_PARTICLE + (REF_FWD_NODE + REF_ADJ_NODE) * r
+
+
+# Implementation bytes (what each kernel of this build reads and writes),
+# per launch: n particles, a active grid nodes, nb bins.
 KERNELS = {
     0: ("sort", lambda n, a, nb: 40 * n + 12 * nb),         # key in/out, start, placed, rank,
                                                              # sorted; counts, cursors, starts
@@ -121,15 +152,16 @@ def _cpu_worker(args):
     import numpy as np
     from oracle import ref_vec as RV
     from oracle import ref_loop as RL
-    pts, G, dx, dt, steps, sched = args
+    pts, G, dx, dt, steps, sched, kind = args
     mat = RL.Material(E=1.0e3, nu=0.3, density=1.0)
     hi = np.ceil(pts.max(0) / dx).astype(int) + 4
     cfg = RL.SimConfig(N=int(hi.max()), dx=dx, dt=dt, sp=dx / 2, material=mat,
                        dims=tuple(int(h) for h in hi))
     st0 = RL.ParticleState.rest(pts)
     st0.Fg = G.copy()
+    model = RL.NeoHookean(mat) if kind == "nh" else RL.Hencky(mat)
     t0 = time.perf_counter()
-    fin, caches = RV.rollout(st0, RL.NeoHookean(mat), cfg, steps, advect=True, sched=sched)
+    fin, caches = RV.rollout(st0, model, cfg, steps, advect=True, sched=sched)
     n = len(pts)
     Fb = np.linalg.det(fin.F)[:, None, None] * np.swapaxes(np.linalg.inv(fin.F), 1, 2)
     RV.rollout_vjp(caches, np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3, 3)), Fb,
@@ -143,7 +175,7 @@ def _cpu_loop_worker(args):
     thread."""
     import numpy as np
     from oracle import ref_loop as RL
-    pts, G, dx, dt, steps, sched = args
+    pts, G, dx, dt, steps, sched, kind = args
     mat = RL.Material(E=1.0e3, nu=0.3, density=1.0)
     hi = np.ceil(pts.max(0) / dx).astype(int) + 4
     cfg = RL.SimConfig(N=int(hi.max()), dx=dx, dt=dt, sp=dx / 2, material=mat,
@@ -151,8 +183,9 @@ def _cpu_loop_worker(args):
     st0 = RL.ParticleState.rest(pts)
     st0.Fg = G.copy()
     n = len(pts)
+    model = RL.NeoHookean(mat) if kind == "nh" else RL.Hencky(mat)
     t0 = time.perf_counter()
-    fin, cache = RL.step(st0, RL.NeoHookean(mat), cfg, advect=True)
+    fin, cache = RL.step(st0, model, cfg, advect=True)
     Fb = np.linalg.det(fin.F)[:, None, None] * np.swapaxes(np.linalg.inv(fin.F), 1, 2)
     RL.step_vjp(cache, np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3, 3)), Fb,
                 np.zeros((n, 3, 3)), advect=True)
@@ -175,12 +208,12 @@ def cpu_baseline(scenes, budget_s):
     """FP64 vectorised numpy oracle on the host: P single-thread processes over
     independent instances, each run for a truncated step count."""
     import multiprocessing as mp
-    P = max(1, min(16, os.cpu_count() or 1))
+    P = max(1, os.cpu_count() or 1)
     steps = 4
     jobs = []
     for s in scenes[: 4 * P]:
         jobs.append((s.pts, s.G, s.cfg.dx, s.cfg.dt, steps,
-                     s.sched[:steps] if s.sched is not None else None))
+                     s.sched[:steps] if s.sched is not None else None, s.model.kind))
     # calibrate the number of jobs to the time budget with one job
     n0, t0 = _cpu_worker(jobs[0])
     per_ps = t0 / n0
@@ -216,7 +249,7 @@ def main():
     import numpy as np
     import torch
     from morphompm_amd import _lib
-    from morphompm_amd.batched import BatchedInverse
+    from morphompm_amd.batched import BatchedInverse, cofactor_rows
     from morphompm_amd.engine import BatchPlan, InstanceSpec, Trajectory, ROW_F
     from morphompm_amd.scenes import batch_instances
     import ctypes
@@ -232,8 +265,9 @@ def main():
     G2P_BYTES = g2p_bytes_per_particle(a.sim_steps, a.checkpoint_every)
     ADJ_PARTICLE_BYTES = adj_particle_bytes_per_particle(a.sim_steps, a.checkpoint_every)
     rates = np.array([s.extra["rate"] for s in scenes])
-    print(f"[bench] {len(scenes)} instances, {n_total} particles, {a.sim_steps} steps, "
-          f"K={a.checkpoint_every}", file=sys.stderr, flush=True)
+    models = sorted({s.model.kind for s in scenes})
+    print(f"[bench] {len(scenes)} instances ({', '.join(models)}), {n_total} particles, "
+          f"{a.sim_steps} steps, K={a.checkpoint_every}", file=sys.stderr, flush=True)
 
     if a.pmc_pass:
         # counters are read per kernel name: no tuning dispatches in this run
@@ -266,28 +300,18 @@ def main():
     # targets: the loss at the true rates (the observed shapes)
     bi.set_theta(rates)
     target, _ = bi.loss_and_grad()
-    target = target.cpu().numpy()
-    theta = 0.8 * rates
+    # the inversion starts every instance at the rates' prior mean
+    theta0 = np.full(len(scenes), 0.5 * (0.02 + 0.2))
+    n_iter = max(50, a.warmup + a.steps)
+    bi.start_inversion(target, theta0, tol=1e-9, hist_cap=1)
 
-    def gn_iteration(theta):
-        bi.set_theta(theta)
-        Lv, dL = (t.cpu().numpy() for t in bi.loss_and_grad())
-        resid = Lv - target
-        ok = np.isfinite(dL) & (np.abs(dL) >= 1e-12)
-        step = np.where(ok, np.clip(resid / np.where(ok, dL, 1.0), -0.5, 0.5), 0.0)
-        theta = theta - step
-        nudge = ~ok & (np.abs(resid) >= 1e-9)
-        theta = theta + np.where(nudge, 1e-3 * np.where(resid < 0, 1.0, -1.0), 0.0)
-        return theta, Lv, dL
-
-    for _ in range(a.warmup):
-        theta, _, _ = gn_iteration(theta)
+    bi.prepare_iteration()           # captures the iteration graph
+    bi.gn_iterate(a.warmup)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record()
-    for _ in range(a.steps):
-        theta, Lv, dL = gn_iteration(theta)
+    bi.gn_iterate(a.steps)
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -298,10 +322,24 @@ def main():
     print(f"[bench] {ms_per:.2f} ms per iteration, {value:.4g} particle-steps/s "
           f"(wall {wall:.2f} s)", file=sys.stderr, flush=True)
 
+    # the rest of the inversion, untimed: every instance's recovered rate
+    bi.gn_iterate(n_iter - a.warmup - a.steps)
+    th = bi.theta.cpu().numpy()
+    its = bi.its.cpu().numpy()
+    done = bi.done.cpu().numpy().astype(bool)
+    err = np.abs(th - rates)
+    inversion = {"iterations": n_iter, "theta0": float(theta0[0]), "tol": 1e-9,
+                 "converged": int(done.sum()), "instances": len(scenes),
+                 "recovered_within_1e-4": int((err < 1e-4).sum()),
+                 "max_abs_err": float(err.max()), "median_abs_err": float(np.median(err)),
+                 "median_iterations": float(np.median(its)), "max_iterations": int(its.max())}
+    print(f"[bench] inversion: {inversion['recovered_within_1e-4']}/{len(scenes)} rates within "
+          f"1e-4 after {n_iter} iterations (max err {err.max():.2e})", file=sys.stderr,
+          flush=True)
+
     # Per-kernel timing: one more iteration per kernel, run eagerly on the
     # same stream with HIP events around each launch of that kernel (event
     # records inside captured graphs are refused on this stack).
-    import ctypes
     L = _lib.lib()
     cap = 2 * a.sim_steps
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * cap)]
@@ -309,11 +347,12 @@ def main():
         e.record()
     torch.cuda.synchronize()
     handles = (ctypes.c_void_p * (2 * cap))(*[e.cuda_event for e in evs])
-    bi.set_theta(theta)
+    bi.set_theta(th)
     ktab = {}
     for kid, (kname, _) in KERNELS.items():
         L.mpm_kernel_timer(kid, handles, cap)
         bi._fwd()
+        bi._loss()
         bi._bwd()
         cnt = L.mpm_kernel_timer_count()
         L.mpm_kernel_timer(-1, None, 0)
@@ -334,9 +373,9 @@ def main():
                      ) * int(np.prod(bi.plan.tile))
     for kname, e in ktab.items():
         kid = e["id"]
-        e["algorithmic_bytes_per_launch"] = KERNELS[kid][1](n_total, active, occ_bins)
-        e["achieved_GBs"] = (e["algorithmic_bytes_per_launch"] / (e["avg_ms"] / 1e3) / 1e9
-                             if e["avg_ms"] > 0 else 0.0)
+        e["implementation_bytes_per_launch"] = KERNELS[kid][1](n_total, active, occ_bins)
+        e["implementation_GBs"] = (e["implementation_bytes_per_launch"] / (e["avg_ms"] / 1e3) / 1e9
+                                   if e["avg_ms"] > 0 else 0.0)
     # the dominant single kernel (the sort entry is a 7-launch sequence)
     dom = max((k for k in ktab if k != "sort"), key=lambda k: ktab[k]["ms_per_iteration"])
     d = ktab[dom]
@@ -346,15 +385,29 @@ def main():
         for line in open(tf):
             if line.startswith("traffic_bytes_per_launch"):
                 traffic = float(line.split()[1])
-    roof = {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
-            "avg_launch_ms": d["avg_ms"], "algorithmic_bytes_per_launch": d["algorithmic_bytes_per_launch"],
-            "launches_timed": d["launches_timed"], "timer": "hip events, eager iteration after the timed region"}
+    if dom == "adj_particle":
+        ref_bytes = REF_ADJ_PARTICLE * n_total
+        ref_model = ("reverse particle work of the reference data model: 432 B per particle "
+                     "(read input state, next-step C, output cotangents, Fg_bar; write input "
+                     "cotangents, Fg_bar)")
+    else:
+        ref_bytes = d["implementation_bytes_per_launch"]
+        ref_model = "implementation bytes (no reference-model share for this kernel)"
+    achieved = ref_bytes / (d["avg_ms"] / 1e3) / 1e9
+    roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "algorithmic_bytes_per_launch": ref_bytes, "byte_model": ref_model,
+            "traffic_over_algorithmic": traffic / ref_bytes if traffic else None,
+            "implementation_bytes_per_launch": d["implementation_bytes_per_launch"],
+            "avg_launch_ms": d["avg_ms"], "launches_timed": d["launches_timed"],
+            "particles_per_launch": n_total,
+            "timer": "hip events, eager iteration after the timed region"}
 
-    # whole-iteration algorithmic bytes (fwd + adjoint, per particle-step)
+    # whole-iteration bytes of the reference data model (fwd + adjoint, per
+    # particle-step; the checkpoint recompute is not counted)
     r = active / n_total
-    b_alg = 660 + 128 * r
-    whole = {"B_alg_per_particle_step": b_alg,
+    b_alg = reference_bytes_per_particle_step(r)
+    whole = {"B_alg_per_particle_step": b_alg, "active_nodes_per_particle": r,
              "achieved_GBs": b_alg * value / 1e9,
              "frac": b_alg * value / 1e9 / HBM_PEAK_GBS}
 
@@ -368,14 +421,12 @@ def main():
                             use_graphs=True, keep_segments=nseg)
         st = bk.traj._stores
         hbm = sum(t.numel() * t.element_size() for store in st for t in store)
-        bk.set_theta(theta)
-        bk.loss_and_grad()                         # capture + warm-up
+        bk.start_inversion(target, theta0, tol=1e-9)
+        bk.gn_iterate(1)                           # capture + warm-up
         torch.cuda.synchronize()
         k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         k0.record()
-        for _ in range(a.steps):
-            bk.set_theta(theta)
-            bk.loss_and_grad()
+        bk.gn_iterate(a.steps)
         k1.record()
         torch.cuda.synchronize()
         ms_k = k0.elapsed_time(k1) / a.steps
@@ -399,14 +450,13 @@ def main():
             tr = Trajectory(p, torch.float32, s.n_steps, True, a.checkpoint_every, s.sched)
             dyn0 = p.pack_dyn([(s.pts, np.zeros((s.n, 3)), np.zeros((s.n, 3, 3)),
                                 np.repeat(np.eye(3)[None], s.n, 0))], torch.float32)
-            G = p.pack_mat([np.eye(3) + theta[i] * s.dG], torch.float32)
+            G = p.pack_mat([np.eye(3) + th[i] * s.dG], torch.float32)
             for rep in range(2):
                 torch.cuda.synchronize()
                 t1 = time.perf_counter()
                 fin = tr.forward(dyn0, G)
                 bar = torch.zeros_like(fin)
                 F = fin[ROW_F:ROW_F + 9].to(torch.float64)
-                from morphompm_amd.batched import cofactor_rows
                 bar[ROW_F:ROW_F + 9] = (cofactor_rows(F)[1] * p.live).float()
                 tr.backward(bar)
                 torch.cuda.synchronize()
@@ -414,19 +464,20 @@ def main():
             tot_ps += s.n * s.n_steps
             tot_t += dtt
         plain = {"per_instance_value": tot_ps / tot_t, "instances_sampled": int(len(idx)),
-                 "median_n": int(np.median([scenes[i].n for i in idx]))}
+                 "median_n": int(np.median([scenes[i].n for i in idx])),
+                 "models_sampled": [scenes[i].model.kind for i in idx]}
 
     # FP32 vs FP64 on the same batch (one forward + adjoint)
     acc = None
     if not a.no_fp64_check:
-        bi.set_theta(theta)
+        bi.set_theta(th)
         L32, g32 = (t.cpu().numpy() for t in bi.loss_and_grad())
         F32 = bi.final_state()[ROW_F:ROW_F + 9].double().clone()
         del bi
         torch.cuda.empty_cache()
         b64 = BatchedInverse(scenes, dtype=torch.float64, tile=tile, block=block,
                              use_graphs=False)
-        b64.set_theta(theta)
+        b64.set_theta(th)
         L64, g64 = (t.cpu().numpy() for t in b64.loss_and_grad())
         F64 = b64.final_state()[ROW_F:ROW_F + 9]
         live = b64.plan.live
@@ -446,11 +497,13 @@ def main():
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_per,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic",
-           "config": {"workload": "batched inverse growth (Gauss-Newton iteration = 200-step "
-                                  "forward + checkpointed adjoint of every instance)",
+           "config": {"workload": "batched inverse growth (Gauss-Newton iteration = growth, "
+                                  "200-step forward, loss, checkpointed adjoint, gradient and "
+                                  "update of every instance, on the device)",
                       "instances": len(scenes), "particles": n_total,
+                      "models": "neo-Hookean free swell (even), Hencky bilayer (odd)",
                       "sim_steps": a.sim_steps, "checkpoint_every": a.checkpoint_every,
-                      "tile": list(tile), "block": block, "hip_graphs": True},
+                      "tile": list(tile), "block": block, "hip_graphs": "one per iteration"},
            "autotune": tune,
            "roofline": roof, "whole_step_roofline": whole, "kernels": ktab,
            "cpu_baseline": cpu,
@@ -458,7 +511,8 @@ def main():
            "no_recompute": keep_all,
            "batched_over_plain": (value / plain["per_instance_value"]) if plain else None,
            "batched_over_cpu": (value / cpu["value"]) if cpu else None,
-           "fp32_vs_fp64": acc}
+           "fp32_vs_fp64": acc,
+           "inversion": inversion}
     print(json.dumps(out))
 
 

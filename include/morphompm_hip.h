@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 6
+#define MPM_ABI_VERSION 7
 #define MPM_SLOT_ALIGN 256 /* particle and node ranges of an instance start at multiples of this */
 
 enum { MPM_OK = 0, MPM_EINVAL = -1, MPM_ELAUNCH = -2, MPM_ENOTSUP = -3 };
@@ -228,6 +228,54 @@ int32_t mpm_step_bwd_chain_f64(const mpm_batch* b, const double* dyn_in, const d
                                double* theta_acc, int32_t advect, int32_t ready,
                                const double* dyn_prev, const int32_t* sortbuf_prev, void* ws,
                                void* stream);
+
+/* ---- Batched growth inversion on the device -------------------------------
+ * The caller side of a Gauss-Newton iteration over every instance of a batch,
+ * so a whole iteration (growth, forward, loss, reverse, gradient, update) runs
+ * without the host and can be captured as one graph.  Sums are in FP64 and in
+ * a fixed order (per 256-slot chunk, then over an instance's chunks), so the
+ * results are bitwise reproducible.  chunk_acc is caller-owned scratch of
+ * n_slots / 256 doubles.  theta, loss, dloss, target: [n_inst] doubles.
+ * dG: [9][n_slots] doubles (the growth direction of each slot). */
+
+/* G = I + theta[i] dG for the live slots of instance i (I for padding).
+ * The reference's parameterisation is Fg = g I
+ * (python/morphompm/integrate.py:31-34 iso_growth_state); this is its
+ * per-instance, per-particle-direction generalisation. */
+int32_t mpm_growth_from_rates_f32(const mpm_batch* b, const double* theta, const double* dG,
+                                  float* G, void* stream);
+int32_t mpm_growth_from_rates_f64(const mpm_batch* b, const double* theta, const double* dG,
+                                  double* G, void* stream);
+
+/* loss[i] = sum over instance i's particles of det F (FP64), and the whole
+ * cotangent state bar [24][n_slots]: det F F^-T in the F rows of live slots,
+ * zero elsewhere.  Replaces loss_detF and the seed Ff_b of dloss_dg
+ * (python/morphompm/integrate.py:41-42,49). */
+int32_t mpm_loss_det_f32(const mpm_batch* b, const float* dyn, float* bar, double* chunk_acc,
+                         double* loss, void* stream);
+int32_t mpm_loss_det_f64(const mpm_batch* b, const double* dyn, double* bar, double* chunk_acc,
+                         double* loss, void* stream);
+
+/* grad[i] = sum over instance i's particles of <dG, G_bar> (FP64): dL/dtheta_i
+ * for G = I + theta_i dG.  Replaces iso_growth_grad
+ * (python/morphompm/integrate.py:37-38), which is the case dG = I. */
+int32_t mpm_growth_grad_f32(const mpm_batch* b, const float* G_bar, const double* dG,
+                            double* chunk_acc, double* grad, void* stream);
+int32_t mpm_growth_grad_f64(const mpm_batch* b, const double* G_bar, const double* dG,
+                            double* chunk_acc, double* grad, void* stream);
+
+/* One damped Gauss-Newton iteration of every instance with done[i] == 0, the
+ * loop body of infer_growth (python/morphompm/diff.py:14-23): |loss - target|
+ * < tol sets done[i]; |dloss| < 1e-12 or non-finite nudges theta by 1e-3
+ * towards the target; else theta -= clip((loss - target) / dloss, +-0.5).
+ * iters[i] counts the iterations instance i took part in (the reference's
+ * returned it + 1).  hist (optional, [n_inst][hist_cap]) receives each new
+ * theta of a Gauss-Newton step at hist_len[i]++ while hist_len[i] < hist_cap
+ * (the reference's history; the caller writes the initial value). */
+int32_t mpm_gauss_newton(int32_t n_inst, const double* loss, const double* dloss,
+                         const double* target, double tol, double* theta, int32_t* done,
+                         int32_t* iters, double* hist, int32_t hist_cap, int32_t* hist_len,
+                         void* stream);
 
 /* Per-launch timing of one kernel (profiling aid, no reference counterpart).
  * `events` holds 2*n_pairs hipEvent_t handles created with timing enabled.

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # to compare two builds on one device; the in-tree build is the default.
 LIB_PATH = os.environ.get("MORPHOMPM_AMD_LIB") or os.path.join(HERE, "_lib", "libmorphompm_hip.so")
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 SLOT_ALIGN = 256
 
 NEO_HOOKEAN, HENCKY, HERSCHEL_BULKLEY = 0, 1, 2
@@ -84,6 +84,18 @@ def lib():
         f = getattr(L, f"mpm_step_bwd_chain_{suf}")
         f.restype = _I32
         f.argtypes = [ctypes.POINTER(MpmBatch)] + [_P] * 11 + [_I32, _I32, _P, _P, _P, _P]
+        f = getattr(L, f"mpm_growth_from_rates_{suf}")
+        f.restype = _I32
+        f.argtypes = [ctypes.POINTER(MpmBatch), _P, _P, _P, _P]
+        f = getattr(L, f"mpm_loss_det_{suf}")
+        f.restype = _I32
+        f.argtypes = [ctypes.POINTER(MpmBatch), _P, _P, _P, _P, _P]
+        f = getattr(L, f"mpm_growth_grad_{suf}")
+        f.restype = _I32
+        f.argtypes = [ctypes.POINTER(MpmBatch), _P, _P, _P, _P, _P]
+    L.mpm_gauss_newton.restype = _I32
+    L.mpm_gauss_newton.argtypes = [_I32, _P, _P, _P, ctypes.c_double, _P, _P, _P, _P, _I32, _P,
+                                   _P]
     L.mpm_kernel_timer.restype = _I32
     L.mpm_kernel_timer.argtypes = [_I32, _P, _I32]
     L.mpm_kernel_timer_count.restype = _I32

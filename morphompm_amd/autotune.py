@@ -6,9 +6,13 @@ kernels (particles per block).  ``autotune`` builds a ``BatchPlan`` for every
 candidate and times, with HIP events, a short checkpointed trajectory of the
 batch's own particles from rest under their growth tensors (chained forward
 steps, a segment recompute, chained reverse steps), keeping the fastest
-median.  Results are cached per *shape class* -- dtype, model
-mix, advection, and log2 buckets of the particle count, the instance count and
-the mean grid size -- in a JSON file, so a workload is tuned once:
+median.  A candidate replaces the default shape (the engine's default tile,
+256-thread blocks) only when it is at least ``MARGIN`` faster: candidates
+within run-to-run noise of each other would otherwise trade places between
+runs.  Results are cached per *shape class* -- dtype, model mix, advection,
+log2 buckets of the particle count, the instance count and the mean grid size,
+and the candidate set when it is not the full one -- in a JSON file, so a
+workload is tuned once:
 
     tile, block = autotune(specs, pts, dtype, advect, G_list).choice
 
@@ -25,9 +29,11 @@ import numpy as np
 import torch
 
 from . import _lib
-from .engine import BatchPlan, Trajectory
+from .engine import BatchPlan, Trajectory, DEFAULT_TILE
 
 CANDIDATE_BLOCKS = (64, 128, 256)
+DEFAULT_BLOCK = 256
+MARGIN = 0.02       # a candidate must beat the default shape by this fraction
 
 
 def cache_path():
@@ -46,6 +52,36 @@ def shape_class(specs, dtype, advect):
     b = lambda v: int(round(math.log2(max(v, 1))))
     return (f"{dt}-m{''.join(map(str, kinds))}-a{int(bool(advect))}"
             f"-n{b(n)}-i{b(len(specs))}-g{b(nodes)}")
+
+
+def cache_key(specs, dtype, advect, tiles, blocks, built):
+    """The shape class, plus the candidate set when it is not every built tile
+    with every candidate block (a restricted tune answers only restricted
+    requests)."""
+    key = shape_class(specs, dtype, advect)
+    if sorted(tiles) != sorted(built) or sorted(blocks) != sorted(CANDIDATE_BLOCKS):
+        key += "|" + ";".join(_label(t, b) for t in sorted(tiles) for b in sorted(blocks))
+    return key
+
+
+def pick(ms, default_label, margin=MARGIN):
+    """The fastest candidate, unless the default is within ``margin`` of it."""
+    best = min(ms, key=ms.get)
+    if default_label in ms and ms[default_label] <= (1.0 + margin) * ms[best]:
+        return default_label
+    return best
+
+
+def lookup(path, key, tiles, blocks, built):
+    """The cached result for ``key`` if its shape is one of the requested
+    candidates and is built, else None."""
+    hit = load_cache(path).get(key)
+    if not hit:
+        return None
+    tile = tuple(hit.get("tile", ()))
+    if tile not in tiles or tile not in built or hit.get("block") not in blocks:
+        return None
+    return TuneResult(tile, int(hit["block"]), hit.get("ms", {}), key, True)
 
 
 def load_cache(path):
@@ -121,14 +157,15 @@ def autotune(specs, pts_list, dtype=torch.float32, advect=True, G_list=None, til
              blocks=CANDIDATE_BLOCKS, steps=6, reps=5, cache=True, path=None):
     """Best (tile, block) for this batch; see the module docstring."""
     _lib.require_gpu()
-    key = shape_class(specs, dtype, advect)
     path = path or cache_path()
     built = [tuple(t) for t in _lib.tile_shapes()]
     tiles = [tuple(t) for t in (tiles or built)]
+    blocks = tuple(int(b) for b in blocks)
+    key = cache_key(specs, dtype, advect, tiles, blocks, built)
     if cache:
-        hit = load_cache(path).get(key)
-        if hit and tuple(hit["tile"]) in built and hit["block"] in blocks:
-            return TuneResult(tuple(hit["tile"]), int(hit["block"]), hit.get("ms", {}), key, True)
+        hit = lookup(path, key, tiles, blocks, built)
+        if hit is not None:
+            return hit
     if G_list is None:
         G_list = [np.repeat((1.05 * np.eye(3))[None], s.n, 0) for s in specs]
     ms = {}
@@ -137,7 +174,7 @@ def autotune(specs, pts_list, dtype=torch.float32, advect=True, G_list=None, til
         for block in blocks:
             ms[_label(tile, block)] = _time_candidate(specs, pts_list, G_list, dtype, advect, tile,
                                                       block, steps, reps)
-    best = min(ms, key=ms.get)
+    best = pick(ms, _label(DEFAULT_TILE, DEFAULT_BLOCK))
     tl, bl = best.split("/")
     res = TuneResult(tuple(int(v) for v in tl.split(",")), int(bl), ms, key, False)
     if cache:

@@ -3,7 +3,10 @@
 ``BatchedInverse`` lays out instances of uneven particle counts in one
 ``BatchPlan`` and runs, per Gauss-Newton iteration, one forward trajectory and
 one checkpointed adjoint over all of them: every kernel launch covers the
-whole batch, and both phases are captured once as HIP graphs and replayed.
+whole batch.  The caller-side steps run on the device too (HIP kernels of the
+library: growth tensors from the rates, the loss and its seed cotangent, the
+gradient reduction and the update), so one whole iteration is captured once as
+a HIP graph and replayed with no host round trip.
 
 Growth parameterisation: instance i has a scalar rate theta_i and per-particle
 direction dG_p, G_p = I + theta_i dG_p, ramped by the instance's schedule.
@@ -12,11 +15,14 @@ at the final step, in float64.  dL_i/dtheta_i = sum_p <dG_p, G_bar_p>.
 The update is the reference's damped Gauss-Newton (python/morphompm/diff.py:9-24)
 applied per instance.
 """
+import ctypes
+
 import numpy as np
 import torch
 
-from .diff import gauss_newton_update
-from .engine import BatchPlan, InstanceSpec, Trajectory, DEFAULT_TILE, ROW_F
+from . import _lib
+from ._lib import SLOT_ALIGN, check, ptr, stream_ptr, suffix
+from .engine import BatchPlan, InstanceSpec, Trajectory, DEFAULT_TILE
 
 
 def cofactor_rows(F):
@@ -76,27 +82,66 @@ class BatchedInverse:
                                  np.repeat(np.eye(3)[None], k, 0)) for s, k in zip(scenes, n)],
                                dtype, self.perm)
         self.dG = p.pack_mat([s.dG for s in scenes], torch.float64, self.perm)
-        self.eye9 = torch.zeros(9, p.n_slots, dtype=torch.float64, device=p.device)
-        self.eye9[[0, 4, 8]] = p.live.to(torch.float64)
-        self.slot_inst = torch.from_numpy(p.slot_inst_np).to(p.device)
-        self.G = p.new_mat(dtype)
+        # the growth kernel writes the trajectory's growth tensor in place
+        self.G = self.traj.G
         self.bar = p.new_dyn(dtype)
+        f64 = dict(dtype=torch.float64, device=p.device)
+        i32 = dict(dtype=torch.int32, device=p.device)
+        ni = p.n_inst
+        self.theta = torch.zeros(ni, **f64)
+        self.loss = torch.zeros(ni, **f64)
+        self.grad = torch.zeros(ni, **f64)
+        self.chunk_acc = torch.zeros(p.n_slots // SLOT_ALIGN, **f64)
+        # Gauss-Newton state (start_inversion)
+        self.target = torch.zeros(ni, **f64)
+        self.done = torch.zeros(ni, **i32)
+        self.its = torch.zeros(ni, **i32)
+        self.hist = torch.zeros(ni, 1, **f64)
+        self.hist_len = torch.zeros(ni, **i32)
+        self.tol = 1e-9
         self.use_graphs = use_graphs
-        self._g_fwd = self._g_bwd = None
+        self._g_fwd = self._g_bwd = self._g_iter = None
+        self._g_iter_key = None
         self._final = None
 
     @property
     def particles(self):
         return sum(s.n for s in self.scenes)
 
-    def set_theta(self, theta):
-        th = torch.as_tensor(np.asarray(theta, dtype=np.float64), device=self.plan.device)
-        G = self.eye9 + th[self.slot_inst][None, :] * self.dG
-        self.G.copy_(G.to(self.dtype))
+    # -- device steps ---------------------------------------------------------
+    def _call(self, name, *args):
+        f = getattr(_lib.lib(), f"{name}_{suffix(self.dtype)}")
+        check(f(ctypes.byref(self.plan.c), *args, stream_ptr()), name)
 
-    # -- the two phases, each one launch sequence over the whole batch --------
+    def _growth(self):
+        self._call("mpm_growth_from_rates", ptr(self.theta), ptr(self.dG), ptr(self.G))
+
     def _fwd(self):
-        self._final = self.traj.forward(self.dyn0, self.G)
+        self._final = self.traj.forward(self.dyn0)
+
+    def _loss(self):
+        self._call("mpm_loss_det", ptr(self._final), ptr(self.bar), ptr(self.chunk_acc),
+                   ptr(self.loss))
+
+    def _bwd(self):
+        self.traj.backward(self.bar)
+
+    def _grad(self):
+        self._call("mpm_growth_grad", ptr(self.traj.G_bar), ptr(self.dG), ptr(self.chunk_acc),
+                   ptr(self.grad))
+
+    def _update(self, state=None):
+        theta, done, its, hist, hist_len = state or (self.theta, self.done, self.its, self.hist,
+                                                     self.hist_len)
+        check(_lib.lib().mpm_gauss_newton(self.plan.n_inst, ptr(self.loss), ptr(self.grad),
+                                          ptr(self.target), float(self.tol), ptr(theta),
+                                          ptr(done), ptr(its), ptr(hist), hist.shape[1],
+                                          ptr(hist_len), stream_ptr()), "gauss_newton")
+
+    def set_theta(self, theta):
+        """Per-instance rates -> the growth tensors of the next forward pass."""
+        self.theta.copy_(torch.as_tensor(np.asarray(theta, dtype=np.float64)))
+        self._growth()
 
     def final_state(self):
         """The final particle state [24, n_slots] of the last forward run.  The
@@ -104,27 +149,59 @@ class BatchedInverse:
         segment when needed (read it before the next forward)."""
         return self.traj.final()
 
-    def _bwd(self):
-        self.traj.backward(self.bar)
-
-    def _capture(self):
+    # -- graphs ---------------------------------------------------------------
+    def _warm(self, fn):
+        """Runs fn once on a side stream outside any capture (first launches
+        load kernels and allocate), then waits for it."""
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):          # warm-up outside capture
-            self._fwd()
-            self._bwd()
+        with torch.cuda.stream(s):
+            fn()
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
+
+    def _capture(self):
+        def phases():
+            self._fwd()
+            self._loss()
+            self._bwd()
+            self._grad()
+        self._warm(phases)
         self._g_fwd = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self._g_fwd):
             self._fwd()
+            self._loss()
         self._seg_after_fwd = self.traj.store_state()
         self._g_bwd = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self._g_bwd):
             self._bwd()
+            self._grad()
         self._seg_after_bwd = self.traj.store_state()
 
+    def _capture_iteration(self):
+        """One whole Gauss-Newton iteration as one graph: growth, forward,
+        loss, reverse, gradient, update.  The warm-up runs the update on a
+        copy of the inversion state, so capturing does not advance it."""
+        def phases(state):
+            self._growth()
+            self._fwd()
+            self._loss()
+            self._bwd()
+            self._grad()
+            self._update(state)
+        scratch = tuple(t.clone() for t in (self.theta, self.done, self.its, self.hist,
+                                            self.hist_len))
+        theta0 = self.theta.clone()
+        self._warm(lambda: phases(scratch))
+        self.theta.copy_(theta0)
+        self._g_iter = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self._g_iter):
+            phases(None)
+        self._seg_after_iter = self.traj.store_state()
+        self._g_iter_key = (self.tol, self.hist.data_ptr(), self.hist.shape[1])
+
     def run_forward(self):
+        """Forward pass and loss of the current growth tensors."""
         if self.use_graphs:
             if self._g_fwd is None:
                 self._capture()
@@ -133,9 +210,11 @@ class BatchedInverse:
             self.traj.set_store_state(self._seg_after_fwd)
         else:
             self._fwd()
+            self._loss()
         return self._final
 
     def run_backward(self):
+        """Adjoint of the last forward pass and the per-instance gradient."""
         if self.use_graphs:
             if self._g_bwd is None:
                 self._capture()
@@ -143,38 +222,68 @@ class BatchedInverse:
             self.traj.set_store_state(self._seg_after_bwd)
         else:
             self._bwd()
+            self._grad()
         return self.traj.G_bar
 
     def loss_and_grad(self):
-        """Per-instance (L_i, dL_i/dtheta_i) as float64 tensors."""
-        p = self.plan
-        fin = self.run_forward()
-        Ff = fin[ROW_F:ROW_F + 9].to(torch.float64)
-        det, cof = cofactor_rows(Ff)
-        loss = p.per_instance_sum(det)
-        self.bar.zero_()
-        self.bar[ROW_F:ROW_F + 9] = (cof * p.live).to(self.dtype)
-        Gb = self.run_backward()
-        g = (Gb.to(torch.float64) * self.dG).sum(0)
-        return loss, p.per_instance_sum(g)
+        """Per-instance (L_i, dL_i/dtheta_i) as float64 device tensors."""
+        self.run_forward()
+        self.run_backward()
+        return self.loss.clone(), self.grad.clone()
+
+    # -- inversion ------------------------------------------------------------
+    def start_inversion(self, targets, theta0, tol=1e-9, hist_cap=1):
+        """Resets the Gauss-Newton state: targets, initial rates (also the first
+        history entry), nothing converged, no iterations."""
+        self.target.copy_(torch.as_tensor(np.asarray(targets, dtype=np.float64)))
+        self.theta.copy_(torch.as_tensor(np.asarray(theta0, dtype=np.float64)))
+        self.tol = float(tol)
+        if self.hist.shape[1] < hist_cap:
+            self.hist = torch.zeros(self.plan.n_inst, int(hist_cap), dtype=torch.float64,
+                                    device=self.plan.device)
+        self.hist.zero_()
+        self.hist[:, 0] = self.theta
+        self.hist_len.fill_(1)
+        self.done.zero_()
+        self.its.zero_()
+
+    def prepare_iteration(self):
+        """Captures the iteration graph now (when use_graphs), so that later
+        gn_iterate calls only replay it."""
+        if self.use_graphs and self._g_iter is None:
+            self._capture_iteration()
+
+    def gn_iterate(self, n=1):
+        """n Gauss-Newton iterations of every unconverged instance, on the
+        device (graph replays when use_graphs); no host synchronisation."""
+        for _ in range(n):
+            if self.use_graphs:
+                key = (self.tol, self.hist.data_ptr(), self.hist.shape[1])
+                if self._g_iter is None or self._g_iter_key != key:
+                    self._capture_iteration()
+                self._g_iter.replay()
+                self.traj.set_store_state(self._seg_after_iter)
+            else:
+                self._growth()
+                self._fwd()
+                self._loss()
+                self._bwd()
+                self._grad()
+                self._update()
 
     def infer(self, targets, theta0, iters=50, tol=1e-9, return_history=False):
-        """Reference Gauss-Newton per instance; all instances are recomputed every
-        iteration (one launch sequence), converged ones keep their theta."""
-        theta = np.array(theta0, dtype=np.float64)
-        targets = np.asarray(targets, dtype=np.float64)
-        done = np.zeros(len(theta), dtype=bool)
-        its = np.zeros(len(theta), dtype=np.int64)
-        hist = [[t] for t in theta]
-        for it in range(iters):
-            self.set_theta(theta)
-            L, dL = (t.cpu().numpy() for t in self.loss_and_grad())
-            for i in np.nonzero(~done)[0]:
-                its[i] = it + 1
-                theta[i], done[i], appended = gauss_newton_update(theta[i], L[i], dL[i],
-                                                                  targets[i], tol)
-                if appended:
-                    hist[i].append(theta[i])
-            if done.all():
+        """Reference Gauss-Newton per instance (diff.py:9-24): all instances are
+        recomputed every iteration (one launch sequence), converged ones keep
+        their theta; stops when every instance has converged.  Returns (theta,
+        iterations[, history]) as numpy."""
+        self.start_inversion(targets, theta0, tol, hist_cap=iters + 1)
+        for _ in range(iters):
+            self.gn_iterate(1)
+            if bool(self.done.all()):
                 break
-        return (theta, its, hist) if return_history else (theta, its)
+        theta = self.theta.cpu().numpy()
+        its = self.its.cpu().numpy().astype(np.int64)
+        if not return_history:
+            return theta, its
+        h, hl = self.hist.cpu().numpy(), self.hist_len.cpu().numpy()
+        return theta, its, [list(h[i, :hl[i]]) for i in range(len(hl))]

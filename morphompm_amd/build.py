@@ -11,6 +11,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_lib")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# one shared library: the step kernels and the batched-inversion kernels
+HIP_SOURCES = ["mpm_kernels.hip", "mpm_inverse.hip"]
 HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
              # multiply-adds fused per source expression only, so a kernel's
              # rounding does not depend on which template instance it is in
@@ -33,7 +35,8 @@ def build(verbose=False):
     os.makedirs(OUT, exist_ok=True)
     hip_out = os.path.join(OUT, "libmorphompm_hip.so")
     tmp = hip_out + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-o", tmp, os.path.join(CSRC, "mpm_kernels.hip")], verbose)
+    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + [os.path.join(CSRC, f) for f in HIP_SOURCES],
+         verbose)
     os.replace(tmp, hip_out)
     host_out = os.path.join(OUT, "libmpm_host.so")
     _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", "-o", host_out + ".tmp",

@@ -31,19 +31,14 @@
 #include <type_traits>
 
 #include "../../include/morphompm_hip.h"
+#include "mpm_internal.h"
 #include "mpm_math.h"
 
 using namespace mpm;
 
-namespace {
+namespace mpm_internal {
 
-constexpr int kChunk = MPM_SLOT_ALIGN;  // slots per instance-table entry
-constexpr int kPayload = 15;            // fx[3], Q[3], A'[9]
-constexpr int kRecord = 16;             // payload record, padded to 4 quads
-constexpr int kScanBlock = 256;
-constexpr int kScanPer = 4;             // elements per thread in the bin scan
-
-thread_local char g_err[512] = "";
+MPM_HIDDEN thread_local char g_err[512] = "";
 
 int set_err(int code, const char* fmt, ...) {
   va_list ap;
@@ -58,6 +53,20 @@ int check_launch(const char* what) {
   if (e != hipSuccess) return set_err(MPM_ELAUNCH, "%s: %s", what, hipGetErrorString(e));
   return MPM_OK;
 }
+
+}  // namespace mpm_internal
+
+namespace {
+
+using mpm_internal::check_launch;
+using mpm_internal::g_err;
+using mpm_internal::set_err;
+
+constexpr int kChunk = MPM_SLOT_ALIGN;  // slots per instance-table entry
+constexpr int kPayload = 15;            // fx[3], Q[3], A'[9]
+constexpr int kRecord = 16;             // payload record, padded to 4 quads
+constexpr int kScanBlock = 256;
+constexpr int kScanPer = 4;             // elements per thread in the bin scan
 
 // Per-instance constants in working precision.
 template <typename T>
@@ -1775,6 +1784,8 @@ int stress_vjp_impl(int32_t model, const double* prm6, int64_t n, int64_t stride
 }
 
 }  // namespace
+
+int mpm_internal::validate_batch(const mpm_batch* b) { return validate(b); }
 
 extern "C" {
 

@@ -6,9 +6,9 @@ stop when |L - target| < tol, nudge g by +-1e-3 when |L'| < 1e-12 or L' is not
 finite; returns (g, iterations[, history]).  L and L' come from the GPU
 ``dloss_dg``.
 
-``infer_growth_batched`` runs the same iteration for many instances at once:
-one forward + adjoint launch sequence per iteration covers the whole batch
-(see ``morphompm_amd.batched``).
+``batched.BatchedInverse.infer`` runs the same iteration for many instances at
+once: one forward + adjoint launch sequence per iteration covers the whole
+batch, and the update itself runs on the device (``mpm_gauss_newton``).
 """
 import numpy as np
 import torch

@@ -9,8 +9,9 @@ Same signatures and defaults as the reference.  ``caches`` is an opaque
 trajectory handle: instead of every step's cache it keeps a checkpoint every
 ``checkpoint_every`` steps (keyword; default: all steps in one segment) and
 ``rollout_vjp`` recomputes one segment at a time.  Keyword extensions:
-``sched`` (per-step growth factors a_t with Fg_t = I + a_t (Fg - I); the Fg
-cotangent returned is then that of Fg) and ``dtype``.
+``sched`` (per-step growth factors a_t with Fg_t = I + a_t (Fg - I); the
+returned state's Fg is then the last step's, and the Fg cotangent returned is
+that of the input Fg) and ``dtype``.
 Loss and gradient sums accumulate in float64.
 """
 import numpy as np
@@ -42,8 +43,14 @@ def rollout(state0, model, cfg, n_steps, advect=False, *, checkpoint_every=None,
     G = plan.pack_mat([state0.Fg], dt)
     fin = traj.forward(dyn0, G)
     x, v, C, F = plan.unpack_dyn(fin)
-    st = ParticleState(like.out(x), like.out(v), like.out(C), like.out(F),
-                       like.out(plan.unpack_mat(G)))
+    Fg = plan.unpack_mat(G)
+    if traj.sched is not None and traj.n_steps > 0:
+        # the last step's growth tensor, I + a (G - I) (a == 1: G itself)
+        a = float(traj.sched[traj.n_steps - 1, 0])
+        if a != 1.0:
+            eye = torch.eye(3, dtype=Fg.dtype, device=Fg.device)
+            Fg = eye + a * (Fg - eye)
+    st = ParticleState(like.out(x), like.out(v), like.out(C), like.out(F), like.out(Fg))
     return st, RolloutHandle(plan, traj, like)
 
 

@@ -100,14 +100,16 @@ def bilayer_strip(ratio=1.08, n_steps=500, seed=0, dims=(96, 32, 32), lo=(16, 8,
 
 
 def batch_instances(count=256, seed=0, n_lo=1000, n_hi=32768, rate_lo=0.02, rate_hi=0.2,
-                    n_steps=200, dx=1.0 / 64, dt=1e-4, margin=4, model="nh"):
+                    n_steps=200, dx=1.0 / 64, dt=1e-4, margin=4, model="mixed"):
     """Batched inverse-growth workload: `count` instances, particle counts
     n_i = 8 ceil(exp(U(ln n_lo, ln n_hi)) / 8), growth rates U(rate_lo,
     rate_hi), one rng (seed 0).  Even i: free-swell cube; odd i: bilayer strip
     (aspect 4 : 1 : 1/2, bottom layer grows).  Cells are filled in lexicographic
     order until n_i / 8 cells; grid = block + `margin` nodes per side
     (4 recorded here; the stencil needs 2).  Free-swell material, dx = 1/64,
-    dt = 1e-4, growth ramped over n_steps."""
+    dt = 1e-4, growth ramped over n_steps.  ``model``: "mixed" (the default:
+    neo-Hookean free swells, Hencky bilayers, as the free-swell and bilayer
+    workloads use them), "nh" or "hencky" for every instance."""
     from . import constitutive as K
     rng = np.random.default_rng(seed)
     ns = 8 * np.ceil(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi), count)) / 8).astype(int)
@@ -137,8 +139,9 @@ def batch_instances(count=256, seed=0, n_lo=1000, n_hi=32768, rate_lo=0.02, rate
             ymid = (margin + 0.5 * bd[1]) * dx
             dG = (pts[:, 1] < ymid).astype(float)[:, None, None] * np.eye(3)
         G = eye + rates[i] * dG
+        kind = model if model != "mixed" else ("nh" if i % 2 == 0 else "hencky")
         scenes.append(Scene("free_swell" if i % 2 == 0 else "bilayer", pts, cfg,
-                            K.NeoHookean(mat) if model == "nh" else K.Hencky(mat), G, dG, n_steps,
+                            K.NeoHookean(mat) if kind == "nh" else K.Hencky(mat), G, dG, n_steps,
                             sched=ramp(n_steps), advect=True, checkpoint_every=20,
                             extra={"rate": rates[i]}))
     return scenes

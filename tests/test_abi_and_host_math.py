@@ -31,7 +31,8 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     lib.mpm_abi_version.restype = ctypes.c_int32
-    assert lib.mpm_abi_version() == 6
+    version = int(re.search(r"#define MPM_ABI_VERSION (\d+)", open(HEADER).read()).group(1))
+    assert lib.mpm_abi_version() == version
 
 
 def test_ctypes_structs_match_c_layout(tmp_path):

@@ -160,6 +160,40 @@ def test_stress_entry_points_reject_bad_arguments(suf):
     assert vj(0, prm, 0, 0, FAKE, FAKE, None, FAKE, FAKE, FAKE, None, None, None) == 0
 
 
+@pytest.mark.parametrize("suf", ["f32", "f64"])
+def test_inversion_entry_points_reject_bad_arguments(suf):
+    L = _lib.lib()
+    b = _batch()
+    f = getattr(L, f"mpm_growth_from_rates_{suf}")
+    assert f(ctypes.byref(_batch(n_slots=300)), FAKE, FAKE, FAKE, None) == MPM_EINVAL
+    assert "multiples" in _msg()
+    assert f(ctypes.byref(b), None, FAKE, FAKE, None) == MPM_EINVAL and "NULL" in _msg()
+    g = getattr(L, f"mpm_loss_det_{suf}")
+    assert g(ctypes.byref(b), FAKE, FAKE + 256, None, FAKE, None) == MPM_EINVAL
+    assert "NULL" in _msg()
+    assert g(ctypes.byref(b), FAKE, FAKE, FAKE + 512, FAKE + 768, None) == MPM_EINVAL
+    assert "alias" in _msg()
+    h = getattr(L, f"mpm_growth_grad_{suf}")
+    assert h(ctypes.byref(b), FAKE, None, FAKE, FAKE, None) == MPM_EINVAL and "NULL" in _msg()
+
+
+def test_gauss_newton_rejects_bad_arguments():
+    L = _lib.lib()
+    args = [4, FAKE, FAKE, FAKE, 1e-9, FAKE, FAKE, FAKE, None, 0, None, None]
+    a = list(args)
+    a[0] = 0
+    assert L.mpm_gauss_newton(*a) == MPM_EINVAL and "n_inst" in _msg()
+    a = list(args)
+    a[5] = None
+    assert L.mpm_gauss_newton(*a) == MPM_EINVAL and "NULL" in _msg()
+    a = list(args)
+    a[8] = FAKE                      # a history with no capacity or length counters
+    assert L.mpm_gauss_newton(*a) == MPM_EINVAL and "hist_cap" in _msg()
+    a = list(args)
+    a[4] = -1.0
+    assert L.mpm_gauss_newton(*a) == MPM_EINVAL and "tol" in _msg()
+
+
 def test_kernel_timer_arguments():
     L = _lib.lib()
     assert L.mpm_kernel_timer(0, None, 0) == MPM_EINVAL and "events" in _msg()

@@ -266,3 +266,66 @@ def test_kept_segments_under_graphs_match_eager():
         torch.cuda.synchronize()
         assert torch.equal(Le, Lk) and torch.equal(ge, gk), rep
         assert torch.equal(eager.final_state(), kept.final_state()), rep
+
+
+def test_iteration_graph_matches_eager_device_loop():
+    """A whole Gauss-Newton iteration (growth, forward, loss, reverse, gradient,
+    update on the device) captured as one HIP graph: every replay, with eager
+    work and synchronisations between replays, leaves the same rates,
+    convergence flags, iteration counts and histories as the eager device
+    loop, bit for bit; the graph is captured without advancing the state."""
+    from morphompm_amd.batched import BatchedInverse
+    sc = _small_batch(("nh", "hencky", "nh"), n_steps=10)
+    rates = np.array([s.extra["rate"] for s in sc])
+    eager = BatchedInverse(sc, dtype=torch.float32, use_graphs=False)
+    graph = BatchedInverse(sc, dtype=torch.float32, use_graphs=True)
+    eager.set_theta(rates)
+    target = eager.loss_and_grad()[0].cpu().numpy()
+    for bi in (eager, graph):
+        bi.start_inversion(target, 0.6 * rates, tol=1e-9, hist_cap=8)
+    graph.prepare_iteration()
+    torch.cuda.synchronize()
+    assert torch.equal(graph.theta.cpu(), torch.tensor(0.6 * rates))
+    assert int(graph.its.sum()) == 0
+    for it in range(5):
+        eager.gn_iterate(1)
+        graph.gn_iterate(1)
+        torch.cuda.synchronize()
+        _ = eager.loss.sum().item()                   # eager work between replays
+        for a, b in ((eager.theta, graph.theta), (eager.done, graph.done),
+                     (eager.its, graph.its), (eager.hist, graph.hist),
+                     (eager.hist_len, graph.hist_len), (eager.loss, graph.loss),
+                     (eager.grad, graph.grad)):
+            assert torch.equal(a, b), it
+    # the iterates approach the true rates
+    assert np.max(np.abs(graph.theta.cpu().numpy() - rates)) < 1e-3 * np.max(rates)
+
+
+def test_device_update_matches_host_rule():
+    """mpm_gauss_newton against diff.gauss_newton_update on every branch:
+    converged, nudge (|L'| < 1e-12, L' = inf, L' = nan, both signs of the
+    residual), a capped step either way, an uncapped step, and a NaN loss."""
+    import ctypes
+    from morphompm_amd import _lib
+    from morphompm_amd._lib import ptr
+    from morphompm_amd.diff import gauss_newton_update
+    L = np.array([1.0, 2.0, 0.5, 2.0, 2.0, 10.0, -10.0, 1.3, np.nan, 1.0 + 1e-10])
+    dL = np.array([2.0, 1e-13, 1e-13, np.inf, np.nan, 1.0, 1.0, 3.0, 2.0, 5.0])
+    tgt = np.ones_like(L)
+    th0 = np.linspace(0.5, 1.5, len(L))
+    dev = lambda a, t=torch.float64: torch.tensor(a, dtype=t, device="cuda")
+    th, done, its = dev(th0), dev(np.zeros(len(L)), torch.int32), dev(np.zeros(len(L)), torch.int32)
+    hist = torch.zeros(len(L), 3, dtype=torch.float64, device="cuda")
+    hist[:, 0] = th
+    hl = torch.ones(len(L), dtype=torch.int32, device="cuda")
+    rc = _lib.lib().mpm_gauss_newton(len(L), ptr(dev(L)), ptr(dev(dL)), ptr(dev(tgt)), 1e-9,
+                                     ptr(th), ptr(done), ptr(its), ptr(hist), 3, ptr(hl),
+                                     _lib.stream_ptr())
+    assert rc == 0
+    torch.cuda.synchronize()
+    for i in range(len(L)):
+        g, d, app = gauss_newton_update(th0[i], L[i], dL[i], tgt[i], 1e-9)
+        assert th[i].item() == g and bool(done[i].item()) == d and its[i].item() == 1, i
+        assert hl[i].item() == (2 if app else 1), i
+        if app:
+            assert hist[i, 1].item() == g, i

@@ -185,6 +185,36 @@ def test_rollout_checkpointed_equals_full_and_vjp():
         assert _rel(a, b) < 1e-10
 
 
+def test_rollout_with_schedule_matches_oracle():
+    """A growth ramp ending below 1: the returned Fg is the last step's
+    I + a (G - I), and rollout_vjp with a nonzero Fg cotangent returns the
+    cotangent of the input G -- both as the oracle's rollout with the same
+    schedule (FP64, <= 1e-10)."""
+    from morphompm_amd import integrate as G
+    from morphompm_amd.config import SimConfig
+    cfg = SimConfig()
+    pts = RL.GATE_POINTS
+    n = len(pts)
+    sched = np.linspace(0.2, 0.8, 6)
+    rng = np.random.default_rng(5)
+    st0 = G.iso_growth_state(pts, 1.3)
+    st0.Fg = st0.Fg + 0.05 * rng.standard_normal((n, 3, 3))
+    bars = (rng.standard_normal((n, 3)), rng.standard_normal((n, 3)),
+            rng.standard_normal((n, 3, 3)), rng.standard_normal((n, 3, 3)),
+            rng.standard_normal((n, 3, 3)))
+    m = _gpu_models()["nh"]
+    st, h = G.rollout(st0, m, cfg, 6, advect=True, checkpoint_every=4, sched=sched)
+    gb = G.rollout_vjp(h, *bars, advect=True)
+    o0 = RL.ParticleState.rest(pts)
+    o0.Fg = st0.Fg.copy()
+    ost, oc = RV.rollout(o0, RL.NeoHookean(cfg.material), cfg, 6, advect=True, sched=sched)
+    ob = RV.rollout_vjp(oc, *bars, advect=True, sched=sched)
+    assert _rel(st.Fg, ost.Fg) < 1e-14
+    assert _rel(st.F, ost.F) < 1e-12
+    for a, b in zip(gb, ob):
+        assert _rel(a, b) < 1e-10
+
+
 def test_determinism_bitwise():
     """Two identical GPU runs are bit-identical (the fixed-order reductions)."""
     from morphompm_amd import integrate as G

@@ -68,8 +68,17 @@ def _plan_and_dyn(dtype, tile=(4, 8, 8), seed=0):
     return plan, dyn
 
 
+# every tile shape the library instantiates (test_built_tiles_are_listed)
+TILES = [(4, 8, 8), (4, 4, 16), (8, 8, 4), (4, 4, 8), (4, 4, 4), (2, 4, 8)]
+
+
+def test_built_tiles_are_listed():
+    from morphompm_amd import _lib
+    assert sorted(_lib.tile_shapes()) == sorted(TILES)
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-@pytest.mark.parametrize("tile", [(4, 8, 8), (4, 4, 8), (4, 4, 4), (2, 4, 8)])
+@pytest.mark.parametrize("tile", TILES)
 def test_sort_by_cell_matches_contract(dtype, tile):
     plan, dyn = _plan_and_dyn(dtype, tile)
     sb = plan.new_sort()
@@ -92,6 +101,45 @@ def test_sort_by_cell_matches_contract(dtype, tile):
         for i in range(plan.n_inst):     # each instance's sorted range is its slot range
             lo, hi = plan.p_off[i], plan.p_off[i + 1] if i + 1 < plan.n_inst else ns
             assert set(srt[lo:hi].tolist()) == set(range(lo, hi))
+
+
+@pytest.mark.parametrize("tile", [(4, 4, 8), (8, 8, 4), (2, 4, 8)])
+def test_host_bin_keys_match_kernel_keys(tile):
+    """engine.bin_keys on a CUDA tensor (the key that orders slots when a plan
+    packs particles) equals the key mpm_sort_by_cell gives each in-grid
+    particle, read back from the sort record (bin of its sorted position,
+    then that bin's unit and place), with the instance's unit offset removed."""
+    from morphompm_amd.engine import bin_keys
+    plan, dyn = _plan_and_dyn(torch.float64, tile, seed=4)
+    sb = plan.new_sort()
+    plan.sort_by_cell(dyn, sb)
+    torch.cuda.synchronize()
+    s = sb.cpu().numpy().astype(np.int64)
+    ns, nb, U = plan.n_slots, plan.n_bins, plan.n_tiles
+    TB = int(np.prod(tile))
+    rank, start = s[:ns], s[2 * ns:2 * ns + nb + 1]
+    active = s[2 * ns + nb + 1 + U:2 * ns + nb + 1 + 2 * U]
+    n_active = int(s[-1])
+    d = dyn.cpu().numpy()
+    checked = 0
+    for i, sp in enumerate(plan.specs):
+        I = plan._insts[i]
+        sl = slice(plan.p_off[i], plan.p_off[i] + sp.n)
+        x = d[0:3, sl].T + np.array(sp.origin) * sp.dx        # absolute positions
+        u = x / sp.dx - 0.5
+        b = np.floor(u).astype(np.int64) + 2
+        inside = np.all((b >= 0) & (b < np.array(sp.shape) + 2), axis=1)
+        clear = np.all(np.abs(u - np.rint(u)) > 1e-9, axis=1)   # away from cell faces
+        keep = inside & clear
+        j = rank[sl][keep]
+        k = np.searchsorted(start[:n_active * TB + 1], j, side="right") - 1
+        kern = active[k // TB] * TB + k % TB - I.t_off * TB
+        host = bin_keys(torch.tensor(x[keep], device="cuda"), sp.dx, sp.shape, tile,
+                        [I.shift[a] for a in range(3)])
+        assert host.is_cuda
+        np.testing.assert_array_equal(host.cpu().numpy(), kern)
+        checked += int(keep.sum())
+    assert checked > 3000
 
 
 def test_sort_moves_with_particles():

@@ -82,19 +82,33 @@ def test_oracle_parity_scene_matches_cpp():
     assert abs(dn - dc) / abs(dc) < 1e-3
 
 
+CPP_FIXTURE = os.path.join(ROOT, "tests", "golden", "cpp_parity_scene.npz")
+
+
 @need_ref
+def test_cpp_fixture_is_the_solver_output():
+    """tests/golden/cpp_parity_scene.npz (written by make_cpp_fixture.py) holds
+    exactly what the reference's C++ solver computes for its parity scene."""
+    pts, Fg, _ = parity_scene()
+    with np.load(CPP_FIXTURE) as d:
+        assert np.array_equal(d["pts"], pts) and np.array_equal(d["Fg"], Fg)
+        xc, Fc = cpp_run(pts, Fg, 12, 0.05, int(d["steps"]), 1e-3)
+        assert np.array_equal(d["x"], xc) and np.array_equal(d["F"], Fc)
+
+
 @pytest.mark.gpu
 def test_gpu_fp32_parity_scene_matches_cpp():
-    """The GPU FP32 trajectory of the same scene against the C++ FP32 solver."""
+    """The GPU FP32 trajectory of the same scene against the C++ FP32 solver's
+    output (the committed fixture: no compiled reference code on the GPU side)."""
     from morphompm_amd import constitutive as K, integrate as G
     from morphompm_amd.config import SimConfig
     from morphompm_amd.state import ParticleState
-    pts, Fg, _ = parity_scene()
+    with np.load(CPP_FIXTURE) as d:
+        pts, Fg, steps, xc, Fc = d["pts"], d["Fg"], int(d["steps"]), d["x"], d["F"]
     cfg = SimConfig(N=12, dx=0.05, dt=1.0e-3, damping=0.05, sp=0.03)
     st = ParticleState.rest(pts.astype(np.float32))
     st.Fg = Fg.astype(np.float32)
-    fin, _ = G.rollout(st, K.Hencky(cfg.material), cfg, 30, advect=True)
-    xc, Fc = cpp_run(pts, Fg, 12, 0.05, 30, 1e-3)
+    fin, _ = G.rollout(st, K.Hencky(cfg.material), cfg, steps, advect=True)
     assert np.abs(fin.F - Fc).max() < 1e-2
     assert abs(np.linalg.det(fin.F).mean() - np.linalg.det(Fc).mean()) / abs(np.linalg.det(Fc).mean()) < 1e-3
     assert np.abs(fin.x - xc).max() < 1e-3
