@@ -25,6 +25,13 @@ from ._lib import SLOT_ALIGN, check, ptr, stream_ptr, suffix
 from .engine import BatchPlan, InstanceSpec, Trajectory, DEFAULT_TILE
 
 
+def _f64(v, device):
+    """Per-instance values (numpy, list or tensor) as a float64 device tensor."""
+    if isinstance(v, torch.Tensor):
+        return v.detach().to(device=device, dtype=torch.float64)
+    return torch.as_tensor(np.asarray(v, dtype=np.float64), device=device)
+
+
 def cofactor_rows(F):
     """F rows [9, n] (row-major 3x3 per column) -> (det F [n], cof F rows [9, n]),
     cof F = det F F^-T, the cotangent of det F."""
@@ -140,7 +147,7 @@ class BatchedInverse:
 
     def set_theta(self, theta):
         """Per-instance rates -> the growth tensors of the next forward pass."""
-        self.theta.copy_(torch.as_tensor(np.asarray(theta, dtype=np.float64)))
+        self.theta.copy_(_f64(theta, self.plan.device))
         self._growth()
 
     def final_state(self):
@@ -235,8 +242,8 @@ class BatchedInverse:
     def start_inversion(self, targets, theta0, tol=1e-9, hist_cap=1):
         """Resets the Gauss-Newton state: targets, initial rates (also the first
         history entry), nothing converged, no iterations."""
-        self.target.copy_(torch.as_tensor(np.asarray(targets, dtype=np.float64)))
-        self.theta.copy_(torch.as_tensor(np.asarray(theta0, dtype=np.float64)))
+        self.target.copy_(_f64(targets, self.plan.device))
+        self.theta.copy_(_f64(theta0, self.plan.device))
         self.tol = float(tol)
         if self.hist.shape[1] < hist_cap:
             self.hist = torch.zeros(self.plan.n_inst, int(hist_cap), dtype=torch.float64,

@@ -318,7 +318,8 @@ def test_device_update_matches_host_rule():
     hist = torch.zeros(len(L), 3, dtype=torch.float64, device="cuda")
     hist[:, 0] = th
     hl = torch.ones(len(L), dtype=torch.int32, device="cuda")
-    rc = _lib.lib().mpm_gauss_newton(len(L), ptr(dev(L)), ptr(dev(dL)), ptr(dev(tgt)), 1e-9,
+    dLoss, dGrad, dTgt = dev(L), dev(dL), dev(tgt)    # held until the kernel has run
+    rc = _lib.lib().mpm_gauss_newton(len(L), ptr(dLoss), ptr(dGrad), ptr(dTgt), 1e-9,
                                      ptr(th), ptr(done), ptr(its), ptr(hist), 3, ptr(hl),
                                      _lib.stream_ptr())
     assert rc == 0
