@@ -254,33 +254,41 @@ def _timed(fn):
 
 def test_bioink_full_size_theta_gradient():
     """The full bioink workload: 2,097,152 particles on 128^3, Herschel-Bulkley,
-    300 steps, checkpoint every 25.  (tau_y, K) gradient: FP64 adjoint vs
-    central FD of the FP64 forward (rel 1e-5); FP32 vs FP64 within 2e-2
-    (measured 8e-3 for tau_y, 1.4e-3 for K).  The FP32 bound is loose
-    because this gradient is not stable to perturbations of the trajectory at
-    that level: the FP64 loss's own central differences scatter by 6e-4 at
-    h = 1e-3, and FP32 rounding shifts when particles cross cells the same
-    way (at the reduced sizes, and for the other workloads, FP32 meets 1e-5).
-    The loss is only C1 in theta (particles cross cells), so the FD error
-    grows with h from the jumps in the second derivative and with 1/h from
-    roundoff of a loss ~6e6: h = 1e-4 (tau_y = 50) and 1e-3 (K = 10) sit
-    between (measured FD vs adjoint over h = 1e-1 .. 1e-5: 9e-2, 6e-5, 6e-4,
-    2e-6, 6e-6 for tau_y; 3e-2, 2e-4, 1e-7, 1e-6, 2e-5 for K)."""
+    300 steps of 1e-4, checkpoint every 25.  (tau_y, K) gradient: FP64 adjoint
+    vs central FD of the FP64 forward (rel 1e-5).
+
+    FP32 does not meet rel 1e-5 against FP64 here, and no precision layout can:
+    at this step size the gradient is not a smooth function of the inputs at
+    that level.  The step is ~50x over the explicit limit rho dx^2 / (6 eta)
+    of the near-rest viscosity eta ~ tau_y m = 5e3, and the trajectory sits in
+    a self-limited instability (eta falls as the shear rate grows).  FP64 runs
+    whose growth tensor carries relative noise as small as 1e-11 move theta_bar
+    by ~1e-2, while FP64 runs that differ only in summation order (another P2G
+    tile shape) agree to ~1e-13.  So the FP32 gradient is checked against the
+    spread of FP64 gradients whose inputs (rest positions and growth tensor)
+    carry FP32-level relative noise 2^-24 (4 seeds): FP32 rounds those inputs
+    the same way.  The loss is only C1 in theta (particles cross cells), so
+    the FD error grows with h from the jumps in the second derivative and with
+    1/h from roundoff of a loss ~6e6: h = 1e-4 (tau_y = 50) and 1e-3 (K = 10)
+    sit between."""
     from morphompm_amd import autograd as AG
     from morphompm_amd.scenes import bioink_blob
     sc = bioink_blob()
     assert sc.n == 2097152 and sc.cfg.shape == (128, 128, 128) and sc.n_steps == 300
 
-    def run(theta, dtype, grad=True):
+    def run(theta, dtype, grad=True, x0=None, G=None, tile=None):
         th = torch.tensor(theta, dtype=dtype, device="cuda", requires_grad=grad)
+        rest = list(_rest(sc, dtype))
+        if x0 is not None:
+            rest[0] = torch.tensor(x0, dtype=dtype, device="cuda")
+        Gt = torch.tensor(sc.G if G is None else G, dtype=dtype, device="cuda")
         with torch.set_grad_enabled(grad):
-            x, v, C, F = AG.rollout(*_rest(sc, dtype), torch.tensor(sc.G, dtype=dtype, device="cuda"),
-                                    sc.model, sc.cfg, sc.n_steps, advect=True,
-                                    checkpoint_every=25, hb_params=th)
+            x, v, C, F = AG.rollout(*rest, Gt, sc.model, sc.cfg, sc.n_steps, advect=True,
+                                    checkpoint_every=25, hb_params=th, tile=tile)
             L = (x.double() ** 2).sum() + torch.linalg.det(F.double()).sum()
         if grad:
             L.backward()
-            return float(L), th.grad.double().cpu().numpy()
+            return float(L.detach()), th.grad.double().cpu().numpy()
         return float(L), None
     (L64, g64), t64 = _timed(lambda: run([50.0, 10.0], torch.float64))
     print(f"bioink FP64 gradient {g64} in {t64:.2f} s", flush=True)
@@ -291,11 +299,28 @@ def test_bioink_full_size_theta_gradient():
         fd = (run(tp, torch.float64, False)[0] - run(tm, torch.float64, False)[0]) / (2 * h)
         print(f"bioink theta[{k}]: adjoint {g64[k]:.10e} FD {fd:.10e}", flush=True)
         assert abs(g64[k] - fd) <= 1e-5 * abs(fd) + 1e-12, (k, g64[k], fd)
+    # FP64 does not depend on the summation order
+    _, gt = run([50.0, 10.0], torch.float64, tile=(8, 8, 4))
+    order = np.abs(gt - g64) / np.abs(g64)
+    print(f"bioink FP64 tile 8x8x4 vs 4x4x8: grad rel {order}", flush=True)
+    assert np.all(order <= 1e-11), order
+    # FP64 with FP32-level input noise
+    spread = []
+    for seed in range(4):
+        r = np.random.default_rng(seed)
+        x0 = sc.pts * (1.0 + 2.0 ** -24 * r.standard_normal(sc.pts.shape))
+        G = sc.G * (1.0 + 2.0 ** -24 * r.standard_normal(sc.G.shape))
+        _, gp = run([50.0, 10.0], torch.float64, x0=x0, G=G)
+        spread.append(np.abs(gp - g64) / np.abs(g64))
+        print(f"bioink FP64, inputs x (1 + 2^-24 xi), seed {seed}: grad rel {spread[-1]}",
+              flush=True)
+    spread = np.max(spread, axis=0)
     (_, g32), t32 = _timed(lambda: run([50.0, 10.0], torch.float32))
     rel = np.abs(g32 - g64) / np.abs(g64)
     print(f"bioink 2.1M x 300: FP32 fwd+adj {sc.n * sc.n_steps / t32:.3e} particle-steps/s "
-          f"({t32:.2f} s), FP64 {t64:.2f} s; grad rel.err FP32 {rel}")
-    assert np.all(rel <= 2e-2), (g32, g64)
+          f"({t32:.2f} s), FP64 {t64:.2f} s; grad rel.err FP32 {rel}, FP64 input-noise spread "
+          f"{spread}")
+    assert np.all(rel <= spread), (rel, spread)
 
 
 def test_shell_full_size_control_grid_fd32():
