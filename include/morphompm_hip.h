@@ -5,7 +5,8 @@
  * One shared library (morphompm_amd/_lib/libmorphompm_hip.so) exports these
  * symbols.  Conventions:
  *   - every buffer is a caller-owned device pointer; the library never
- *     allocates device memory;
+ *     allocates device memory (mpm_step_fwd_noalloc_f32 uses a fixed scratch
+ *     array the library holds on each GPU instead of a workspace argument);
  *   - calls are stream-ordered and asynchronous on `stream` (a hipStream_t,
  *     passed as void*; NULL is the default stream);
  *   - functions return MPM_OK (0) or a negative MPM_E* code and never throw or
@@ -158,6 +159,20 @@ int32_t mpm_step_fwd_f32(const mpm_batch* b, const float* dyn_in, const float* F
 int32_t mpm_step_fwd_f64(const mpm_batch* b, const double* dyn_in, const double* Fg,
                          const double* sched, double* dyn_out, double* grid, int32_t* sortbuf,
                          double* payload, int32_t advect, void* ws, void* stream);
+
+/* mpm_step_fwd_f32 with no workspace argument, for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * step's scratch is a 64 MiB array the library holds on each GPU it runs on
+ * until the process ends; a batch whose forward workspace
+ * (mpm_workspace_bytes up to the tile partials) exceeds it is rejected with
+ * MPM_EINVAL before any device work.  Calls allocate nothing.  All calls on
+ * one GPU share its array: calls that overlap on one GPU race.  Results are
+ * bit for bit those of mpm_step_fwd_f32. */
+int32_t mpm_step_fwd_noalloc_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
+                                 const float* sched, float* dyn_out, float* grid,
+                                 int32_t* sortbuf, float* payload, int32_t advect,
+                                 void* stream);
 
 /* The same step for a chain of steps on one workspace and stream, with the
  * next step's P2G payload produced by this step's G2P (the per-particle prep

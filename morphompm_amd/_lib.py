@@ -93,6 +93,9 @@ def lib():
         f = getattr(L, f"mpm_growth_grad_{suf}")
         f.restype = _I32
         f.argtypes = [ctypes.POINTER(MpmBatch), _P, _P, _P, _P, _P]
+    L.mpm_step_fwd_noalloc_f32.restype = _I32
+    L.mpm_step_fwd_noalloc_f32.argtypes = [ctypes.POINTER(MpmBatch), _P, _P, _P, _P, _P, _P, _P,
+                                           _I32, _P]
     L.mpm_gauss_newton.restype = _I32
     L.mpm_gauss_newton.argtypes = [_I32, _P, _P, _P, ctypes.c_double, _P, _P, _P, _P, _I32, _P,
                                    _P]

@@ -28,6 +28,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "../../include/morphompm_hip.h"
@@ -1428,7 +1429,11 @@ int64_t scan_blocks(int64_t n_bins) {
   return (n_bins + kScanBlock * kScanPer - 1) / (kScanBlock * kScanPer);
 }
 
-int64_t carve(const mpm_batch* b, size_t tsz, char* base, Workspace* w) {
+// The regions a forward step uses (sort scratch and tile partials) come
+// first; *fwd_bytes (optional) receives their extent.  The reverse step's
+// regions (two adjoint payload areas, the adjoint grid) follow.
+int64_t carve(const mpm_batch* b, size_t tsz, char* base, Workspace* w,
+              int64_t* fwd_bytes = nullptr) {
   int64_t off = 0;
   auto take = [&](int64_t bytes) {
     char* p = base ? base + off : nullptr;
@@ -1443,8 +1448,9 @@ int64_t carve(const mpm_batch* b, size_t tsz, char* base, Workspace* w) {
   ws.unit_flag = (int32_t*)take(4 * b->n_tiles);
   ws.ucount = (int32_t*)take(4 * ((b->n_tiles + 255) / 256));
   ws.placed = (int32_t*)take(4 * b->n_slots);
-  ws.payload = take((int64_t)tsz * kRecord * b->n_slots);
   ws.partial = take((int64_t)tsz * 4 * tile_nodes(b) * b->n_tiles);
+  if (fwd_bytes) *fwd_bytes = off;
+  ws.payload = take((int64_t)tsz * kRecord * b->n_slots);
   ws.gadj = take((int64_t)tsz * 4 * b->n_nodes);
   ws.payload2 = take((int64_t)tsz * kRecord * b->n_slots);
   if (w) *w = ws;
@@ -1783,6 +1789,44 @@ int stress_vjp_impl(int32_t model, const double* prm6, int64_t n, int64_t stride
   return check_launch("stress_vjp");
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free forward step (mpm_step_fwd_noalloc_f32): a
+// fixed device array, one copy per GPU, holding the forward step's workspace
+// regions (sort scratch and tile partials).  Its device address on each GPU
+// comes from hipGetSymbolAddress, looked up once per device on the first
+// call there; host code never passes the array's own name to a kernel.
+// Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+// (other streams, other threads) race.  Each call clears the counters and
+// flags it needs before use (a fill kernel, not a memset node).
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocScratchBytes = int64_t(64) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_on_current_device(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_err(MPM_ELAUNCH, "step_fwd_noalloc: hipGetDevice: %s",
+                                      hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_err(MPM_ENOTSUP, "step_fwd_noalloc: device %d beyond the %d supported", dev,
+                   kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return set_err(MPM_ELAUNCH, "step_fwd_noalloc: hipGetSymbolAddress(scratch): %s",
+                     hipGetErrorString(e));
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return MPM_OK;
+}
+
 }  // namespace
 
 int mpm_internal::validate_batch(const mpm_batch* b) { return validate(b); }
@@ -1875,6 +1919,27 @@ int32_t mpm_step_fwd_f64(const mpm_batch* b, const double* dyn_in, const double*
   return step_fwd_impl<double>(b, dyn_in, Fg, sched, nullptr, dyn_out, grid, sortbuf, payload,
                                nullptr, advect, 0, MPM_OUT_FULL, ws, stream);
 }
+int32_t mpm_step_fwd_noalloc_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
+                                 const float* sched, float* dyn_out, float* grid,
+                                 int32_t* sortbuf, float* payload, int32_t advect,
+                                 void* stream) {
+  int rc = validate(b);
+  if (rc) return rc;
+  int64_t need = 0;
+  carve(b, sizeof(float), nullptr, nullptr, &need);
+  if (need > kNoAllocScratchBytes)
+    return set_err(MPM_EINVAL,
+                   "step_fwd_noalloc: this batch needs %lld bytes of scratch, more than the "
+                   "%lld the library holds; use mpm_step_fwd_f32 with a workspace",
+                   (long long)need, (long long)kNoAllocScratchBytes);
+  void* ws = nullptr;
+  rc = scratch_on_current_device(&ws);
+  if (rc) return rc;
+  // ready = 0: the step clears the bin counters and unit flags it uses first
+  return step_fwd_impl<float>(b, dyn_in, Fg, sched, nullptr, dyn_out, grid, sortbuf, payload,
+                              nullptr, advect, 0, MPM_OUT_FULL, ws, stream);
+}
+
 int32_t mpm_step_fwd_chain_f32(const mpm_batch* b, const float* dyn_in, const float* Fg,
                                const float* sched, const float* sched_next, float* dyn_out,
                                float* grid, int32_t* sortbuf, float* payload, float* payload_next,

@@ -177,6 +177,18 @@ def test_inversion_entry_points_reject_bad_arguments(suf):
     assert h(ctypes.byref(b), FAKE, None, FAKE, FAKE, None) == MPM_EINVAL and "NULL" in _msg()
 
 
+def test_noalloc_step_rejects_batches_beyond_its_scratch():
+    """The allocation-free forward step refuses, before any device call, a
+    batch whose forward workspace exceeds the library's 64 MiB scratch."""
+    L = _lib.lib()
+    big = _batch(n_slots=1 << 24)                  # 2 x 64 MiB of sort keys alone
+    args = [FAKE, FAKE + 256, None, FAKE + 512, FAKE + 768, FAKE + 1024, FAKE + 1280, 1, None]
+    assert L.mpm_step_fwd_noalloc_f32(ctypes.byref(big), *args) == MPM_EINVAL
+    assert "scratch" in _msg() and str(64 << 20) in _msg()
+    assert L.mpm_step_fwd_noalloc_f32(ctypes.byref(_batch(block=7)), *args) == MPM_EINVAL
+    assert "block" in _msg()
+
+
 def test_gauss_newton_rejects_bad_arguments():
     L = _lib.lib()
     args = [4, FAKE, FAKE, FAKE, 1e-9, FAKE, FAKE, FAKE, None, 0, None, None]
