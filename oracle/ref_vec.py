@@ -315,3 +315,46 @@ def dloss_dg(pts, model, cfg, g, n_steps, advect):
     z3, z33 = np.zeros((n, 3)), np.zeros((n, 3, 3))
     Fg_b = rollout_vjp(caches, z3, z3, z33, Ff_b, z33, advect=advect)[4]
     return loss_detF(st.F), float(np.sum(_tr(Fg_b)))
+
+
+def det_loss_grad_checkpointed(state0, model, cfg, n_steps, dG, advect=True, sched=None, K=20):
+    """L = sum_p det F_final and dL/dtheta = sum_p <dG_p, G_bar_p> for a growth
+    tensor G = state0.Fg moving along dG, from ``rollout`` and ``rollout_vjp``
+    restated with a checkpoint every K steps: the forward pass keeps only the
+    segment-start states, the reverse pass recomputes one segment's caches at
+    a time.  Same steps and the same schedule rule (G_bar = sum_t a_t Fg_bar_t)
+    as rollout / rollout_vjp, so the results are theirs bit for bit, at memory
+    for K caches instead of n_steps.  Returns (L, dL/dtheta, final state)."""
+    G = state0.Fg
+
+    def at(st, t):
+        return ParticleState(st.x, st.v, st.C, st.F, grown(G, sched[t])) if sched is not None \
+            else st
+
+    ckpt = []
+    st = state0
+    for t in range(n_steps):
+        if t % K == 0:
+            ckpt.append(st)
+        st, _ = step(at(st, t), model, cfg, advect=advect)
+    fin = st
+    n = fin.n
+    det = np.linalg.det(fin.F)
+    bars = (np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3, 3)),
+            det[:, None, None] * _T(np.linalg.inv(fin.F)), np.zeros((n, 3, 3)))
+    Gb = np.zeros((n, 3, 3))
+    for s in range((n_steps - 1) // K, -1, -1):
+        t0, t1 = s * K, min(n_steps, (s + 1) * K)
+        caches, st = [], ckpt[s]
+        for t in range(t0, t1):
+            st, c = step(at(st, t), model, cfg, advect=advect)
+            caches.append(c)
+        for t in range(t1 - 1, t0 - 1, -1):
+            if sched is None:
+                bars = step_vjp(caches[t - t0], *bars, advect=advect)
+            else:
+                bars = step_vjp(caches[t - t0], *bars[:4], np.zeros_like(Gb), advect=advect)
+                Gb = Gb + sched[t] * bars[4]
+    if sched is None:
+        Gb = bars[4]
+    return float(np.sum(det)), float(np.sum(Gb * dG)), fin

@@ -309,3 +309,29 @@ def test_planted_mistakes_fail_the_pins(golden, monkeypatch):
     xb = RV.step_vjp(cache, *[golden[f"{key}_bar_{f}"] for f in FIELDS], advect=True)[0]
     monkeypatch.undo()
     assert not _close(xb, golden[f"{key}_vjp_x"], rtol=1e-6, atol=1e-8)
+
+
+def test_checkpointed_det_loss_grad_equals_stored_caches():
+    """ref_vec.det_loss_grad_checkpointed (checkpoint every K steps, recompute
+    per segment) gives rollout + rollout_vjp's loss, growth gradient and final
+    state bit for bit, with and without a schedule, for segment lengths that
+    do and do not divide the step count."""
+    from morphompm_amd.scenes import batch_instances
+    sc = batch_instances(count=2, seed=5, n_lo=200, n_hi=400, n_steps=7)
+    for s in sc:
+        m = RL.NeoHookean(RL.Material()) if s.model.kind == "nh" else RL.Hencky(RL.Material())
+        cfg = RL.SimConfig(N=s.cfg.N, dx=s.cfg.dx, dt=s.cfg.dt, damping=s.cfg.damping,
+                           sp=s.cfg.sp, material=RL.Material(), dims=s.cfg.dims)
+        for sched in (s.sched, None):
+            st0 = RL.ParticleState.rest(s.pts)
+            st0.Fg = np.eye(3) + 0.1 * s.dG
+            fin, caches = RV.rollout(st0, m, cfg, 7, advect=True, sched=sched)
+            det = np.linalg.det(fin.F)
+            n = s.n
+            Fb = det[:, None, None] * np.swapaxes(np.linalg.inv(fin.F), 1, 2)
+            z3, z33 = np.zeros((n, 3)), np.zeros((n, 3, 3))
+            Gb = RV.rollout_vjp(caches, z3, z3, z33, Fb, z33, advect=True, sched=sched)[4]
+            for K in (1, 3, 7):
+                L, g, f2 = RV.det_loss_grad_checkpointed(st0, m, cfg, 7, s.dG, True, sched, K)
+                assert L == det.sum() and g == float(np.sum(Gb * s.dG))
+                assert np.array_equal(f2.F, fin.F) and np.array_equal(f2.x, fin.x)
