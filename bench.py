@@ -56,9 +56,9 @@ def g2p_bytes_per_particle(n_steps, K):
             out = "full" if last else ("record" if keep else "next")
             total += launch(out, not last)
             launches += 1
-    for s, L in enumerate(seg_len[:-1]):                  # recompute of the kept segments
-        for k in range(L):
-            total += launch("record", k < L - 1)
+    for s, L in enumerate(seg_len[:-1]):                  # recompute of the other segments
+        for k in range(L - 1):                            # (the last step runs no G2P)
+            total += launch("record", True)
             launches += 1
     return total / launches
 

@@ -54,8 +54,12 @@ enum { MPM_NEO_HOOKEAN = 0, MPM_HENCKY = 1, MPM_HERSCHEL_BULKLEY = 2 };
 /* Rows of dyn_out a chained step writes (mpm_step_fwd_chain_*): all of them
  * (a checkpoint or a returned state), all but v (a step record: the reverse
  * pass reads x, C and F of it), or x and F only (a state read by the next
- * chained step alone).  Rows not written keep their previous contents. */
-enum { MPM_OUT_FULL = 0, MPM_OUT_RECORD = 1, MPM_OUT_NEXT = 2 };
+ * chained step alone).  Rows not written keep their previous contents.
+ * MPM_OUT_NONE runs no G2P at all: the step writes its record (grid, sort
+ * record) and not dyn_out, which may be NULL -- the last step of a
+ * recomputed checkpoint segment, whose output state is the next checkpoint
+ * already held. */
+enum { MPM_OUT_FULL = 0, MPM_OUT_RECORD = 1, MPM_OUT_NEXT = 2, MPM_OUT_NONE = 3 };
 
 /* One independent simulation (python/morphompm/config.py:21-40 SimConfig plus
  * its constitutive model).  Node grid nx*ny*nz covers [0, n*dx) per axis.

@@ -17,7 +17,8 @@ ABI_VERSION = 7
 SLOT_ALIGN = 256
 
 NEO_HOOKEAN, HENCKY, HERSCHEL_BULKLEY = 0, 1, 2
-OUT_FULL, OUT_RECORD, OUT_NEXT = 0, 1, 2   # rows of dyn_out a chained step writes
+# rows of dyn_out a chained step writes; OUT_NONE: no G2P (the step record only)
+OUT_FULL, OUT_RECORD, OUT_NEXT, OUT_NONE = 0, 1, 2, 3
 
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32

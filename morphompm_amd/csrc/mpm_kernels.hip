@@ -1662,9 +1662,14 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
                   T* payload_next, int advect, int ready, int out, void* ws, void* stream) {
   int rc = validate(b);
   if (rc) return rc;
-  if (out != MPM_OUT_FULL && out != MPM_OUT_RECORD && out != MPM_OUT_NEXT)
-    return set_err(MPM_EINVAL, "step_fwd: out must be MPM_OUT_FULL, _RECORD or _NEXT (got %d)", out);
-  if (!dyn_in || !Fg || !dyn_out || !grid || !sortbuf || !payload || !ws)
+  if (out != MPM_OUT_FULL && out != MPM_OUT_RECORD && out != MPM_OUT_NEXT &&
+      out != MPM_OUT_NONE)
+    return set_err(MPM_EINVAL,
+                   "step_fwd: out must be MPM_OUT_FULL, _RECORD, _NEXT or _NONE (got %d)", out);
+  if (out == MPM_OUT_NONE && payload_next)
+    return set_err(MPM_EINVAL, "step_fwd: MPM_OUT_NONE ends a chain (payload_next must be NULL)");
+  if (!dyn_in || !Fg || (!dyn_out && out != MPM_OUT_NONE) || !grid || !sortbuf || !payload ||
+      !ws)
     return set_err(MPM_EINVAL, "step_fwd: NULL buffer");
   if (dyn_in == dyn_out) return set_err(MPM_EINVAL, "step_fwd: dyn_out must not alias dyn_in");
   if (payload_next == payload) return set_err(MPM_EINVAL, "step_fwd: payload_next aliases payload");
@@ -1686,6 +1691,7 @@ int step_fwd_impl(const mpm_batch* b, const T* dyn_in, const T* Fg, const T* sch
   if (rc) return rc;
   const SortView<const int32_t> sv(b, sortbuf);
   dispatch_fwd<T>(tile_index(b), b, sv, w, payload, grid, st);
+  if (out == MPM_OUT_NONE) return check_launch("step_fwd");   // the step record only
   TimedLaunch tl(MPM_K_G2P, st);
   if (payload_next)
     per_model(b, bs, [&](auto M, const int32_t* ch, dim3 g) {

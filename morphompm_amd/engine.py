@@ -497,21 +497,23 @@ class Trajectory:
     def _src(self, s, k):
         return self.ckpt[s] if k == 0 else self._dst(s, k - 1)
 
-    def _run_segment(self, s, keep):
+    def _run_segment(self, s, keep, recompute=False):
         """The segment's steps as one chain: each step's G2P writes the next
         step's payload and sort keys; the first step computes its own.
         States written: checkpoints and the final state in full; a kept
         segment's step records without v (the reverse pass reads x, C, F);
         the intermediate states of a segment that is not kept with x and F
-        only (the next step reads nothing else).  A recomputed segment
-        rewrites its closing checkpoint with the same bits."""
+        only (the next step reads nothing else).  A segment the reverse pass
+        recomputes (``recompute``) stops its last step after the step record:
+        its output state is the closing checkpoint, already held (the final
+        state, after the last segment, is written as usual)."""
         t0, L = self._seg(s)
         m = self._store(s)
         _, grid, sort, payload = self._stores[m]
         for k in range(L):
             last = k == L - 1
             if last:
-                out = _lib.OUT_FULL
+                out = _lib.OUT_NONE if recompute and s < self.nseg - 1 else _lib.OUT_FULL
             else:
                 out = _lib.OUT_RECORD if keep else _lib.OUT_NEXT
             self.plan.step_fwd_chain(self._src(s, k), self.G, self._a(t0 + k),
@@ -562,7 +564,7 @@ class Trajectory:
         for s in range(self.nseg - 1, -1, -1):
             m = self._store(s)
             if self._held[m] != s:
-                self._run_segment(s, keep=True)
+                self._run_segment(s, keep=True, recompute=True)
             t0, L = self._seg(s)
             _, grid, sort, payload = self._stores[m]
             # one chain per segment: each reverse step's VJP kernel prepares

@@ -110,9 +110,15 @@ def test_step_fwd_rejects_bad_buffers(suf):
             FAKE + 1280, 1, 0, 0, FAKE + 4096, None]                    # payload_next == payload
     assert g(ctypes.byref(b), *args) == MPM_EINVAL and "payload_next" in _msg()
     args[8] = FAKE + 1536
-    for out in (-1, 3):                                               # MPM_OUT_* only
+    for out in (-1, 4):                                               # MPM_OUT_* only
         args[11] = out
         assert g(ctypes.byref(b), *args) == MPM_EINVAL and "MPM_OUT" in _msg()
+    args[11] = 3                                      # MPM_OUT_NONE ends a chain
+    assert g(ctypes.byref(b), *args) == MPM_EINVAL and "payload_next must be NULL" in _msg()
+    args[4] = None                                    # without G2P, dyn_out may be NULL ...
+    args[8] = None
+    args[11] = 2                                      # ... but not otherwise
+    assert g(ctypes.byref(b), *args) == MPM_EINVAL and "NULL buffer" in _msg()
 
 
 @pytest.mark.parametrize("suf", ["f32", "f64"])

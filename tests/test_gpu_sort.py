@@ -162,8 +162,9 @@ def test_sort_moves_with_particles():
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_chained_step_output_rows(dtype):
     """mpm_step_fwd_chain's `out`: MPM_OUT_RECORD leaves the v rows of dyn_out
-    as they were, MPM_OUT_NEXT leaves v and C; the rows written, the grid, the
-    sort record and the next payload equal the full write bit for bit."""
+    as they were, MPM_OUT_NEXT leaves v and C, MPM_OUT_NONE every row; the
+    rows written, the grid, the sort record and the next payload equal the
+    full write bit for bit."""
     from morphompm_amd import _lib
     from morphompm_amd import constitutive as K
     from morphompm_amd.engine import BatchPlan, InstanceSpec, ROW_C, ROW_F, ROW_V, ROW_X
@@ -183,6 +184,18 @@ def test_chained_step_output_rows(dtype):
                          out)
         torch.cuda.synchronize()
         res[out] = (dyn_out, grid, sort, pay_next, p.live)
+    # MPM_OUT_NONE: the step record alone (no G2P; dyn_out untouched)
+    p = BatchPlan([InstanceSpec.from_cfg(sc.n, sc.cfg, m, sc.pts)])
+    dyn = p.pack_dyn([(sc.pts, 0.01 * np.ones((sc.n, 3)), np.zeros((sc.n, 3, 3)),
+                       np.repeat(np.eye(3)[None], sc.n, 0))], dtype)
+    G = p.pack_mat([np.repeat(1.05 * np.eye(3)[None], sc.n, 0)], dtype)
+    dyn_none = torch.full_like(dyn, 7.0)
+    grid, sort, pay = p.new_grid(dtype), p.new_sort(), p.new_payload(dtype)
+    p.step_fwd_chain(dyn, G, None, None, dyn_none, grid, sort, pay, None, True, False,
+                     _lib.OUT_NONE)
+    torch.cuda.synchronize()
+    assert torch.all(dyn_none == 7.0)
+    assert torch.equal(grid, res[_lib.OUT_FULL][1]) and torch.equal(sort, res[_lib.OUT_FULL][2])
     full, live = res[_lib.OUT_FULL][0], res[_lib.OUT_FULL][4]
     assert not torch.any(full[:, live] == 7.0)
     for out, (d, g, srt, pn, _) in res.items():
