@@ -258,6 +258,159 @@ MPM_HD bool near_repeated(const float beta[3]) {
   return g < kNearRepeated * s;
 }
 
+// ---------------------------------------------------------------------------
+// FP32 Hencky through the matrix logarithm, with no eigen-decomposition.
+// tau = mu ln b + (lam/2) tr(ln b) I with b = Fe Fe^T.  With s = tr b / 3 and
+// b' = b / s, ln b = ln(s) I + ln b', and ln b' = 2 artanh(Z) with
+// Z = (b' - I)(b' + I)^{-1}, summed as Gregory's series
+// ln((1 + z) / (1 - z)) = 2 sum_k z^{2k+1} / (2k + 1) through k = kLogTerms,
+// by Horner's rule in Y = Z^2.  Every matrix of the series is a function of
+// b', so they all commute and are symmetric (packed storage).  The first term
+// left out is below |Z|^{2N+2} / (2N+3) relative; with N = 4 and |Z|_F <= 0.2
+// (the eigenvalues of b within a factor 1.5 of their mean) that is < 1e-8,
+// under FP32 resolution.  Outside that range the eigen path runs.
+// The VJP is the reverse-mode derivative of the same steps: the Daleckii-Krein
+// derivative of ln without its divided differences, so repeated eigenvalues
+// need no separate branch.
+// ---------------------------------------------------------------------------
+#ifndef MPM_HENCKY_SERIES
+#define MPM_HENCKY_SERIES 1
+#endif
+constexpr int kLogTerms = 4;
+constexpr float kLogZ2Max = 0.04f;   // |Z|_F^2 bound of the series
+
+struct S3 {  // packed symmetric 3x3
+  float xx, yy, zz, xy, xz, yz;
+};
+
+// A B for commuting symmetric A, B (the product is symmetric).
+MPM_HD S3 s3_mul(const S3& A, const S3& B) {
+  S3 C;
+  C.xx = A.xx * B.xx + A.xy * B.xy + A.xz * B.xz;
+  C.yy = A.xy * B.xy + A.yy * B.yy + A.yz * B.yz;
+  C.zz = A.xz * B.xz + A.yz * B.yz + A.zz * B.zz;
+  C.xy = A.xx * B.xy + A.xy * B.yy + A.xz * B.yz;
+  C.xz = A.xx * B.xz + A.xy * B.yz + A.xz * B.zz;
+  C.yz = A.xy * B.xz + A.yy * B.yz + A.yz * B.zz;
+  return C;
+}
+MPM_HD S3 s3_inv(const S3& A) {
+  const float c00 = A.yy * A.zz - A.yz * A.yz, c01 = A.xz * A.yz - A.xy * A.zz,
+              c02 = A.xy * A.yz - A.xz * A.yy, c11 = A.xx * A.zz - A.xz * A.xz,
+              c12 = A.xy * A.xz - A.xx * A.yz, c22 = A.xx * A.yy - A.xy * A.xy;
+  const float id = 1.0f / (A.xx * c00 + A.xy * c01 + A.xz * c02);
+  return {c00 * id, c11 * id, c22 * id, c01 * id, c02 * id, c12 * id};
+}
+MPM_HD M3<float> s3_full(const S3& A) {
+  M3<float> M;
+  M.a[0] = A.xx; M.a[1] = A.xy; M.a[2] = A.xz;
+  M.a[3] = A.xy; M.a[4] = A.yy; M.a[5] = A.yz;
+  M.a[6] = A.xz; M.a[7] = A.yz; M.a[8] = A.zz;
+  return M;
+}
+
+// The series' intermediates: s, W = (I + b')^{-1}, Z, Y = Z^2 and the Horner
+// partial sums P[k] = sum_{m=k..N} Y^{m-k} / (2m + 1) for k < N.
+struct LogSeries {
+  float s;
+  S3 W, Z, Y, P[kLogTerms];
+};
+
+// ln b = ln(s) I + A; false when b is outside the series' range.
+MPM_HD bool log_series(const M3<float>& b, LogSeries& L, S3& A) {
+  const float s = (b.a[0] + b.a[4] + b.a[8]) * (1.0f / 3.0f);
+  if (!(s > 0.0f && s < 1.0e30f)) return false;
+  const float r = 1.0f / s;
+  const S3 E = {b.a[0] * r - 1.0f, b.a[4] * r - 1.0f, b.a[8] * r - 1.0f,
+                b.a[1] * r, b.a[2] * r, b.a[5] * r};
+  L.W = s3_inv(S3{E.xx + 2.0f, E.yy + 2.0f, E.zz + 2.0f, E.xy, E.xz, E.yz});
+  L.Z = s3_mul(E, L.W);
+  const S3& Z = L.Z;
+  const float z2 = Z.xx * Z.xx + Z.yy * Z.yy + Z.zz * Z.zz +
+                   2.0f * (Z.xy * Z.xy + Z.xz * Z.xz + Z.yz * Z.yz);
+  if (!(z2 <= kLogZ2Max)) return false;  // also NaN
+  L.Y = s3_mul(Z, Z);
+  const float aN = 1.0f / (2 * kLogTerms + 1);
+  S3 P = {aN, aN, aN, 0.0f, 0.0f, 0.0f};
+  for (int k = kLogTerms - 1; k >= 0; --k) {
+    P = s3_mul(L.Y, P);
+    const float ak = 1.0f / (2 * k + 1);
+    P.xx += ak;
+    P.yy += ak;
+    P.zz += ak;
+    L.P[k] = P;
+  }
+  const S3 ZP = s3_mul(Z, P);
+  A = {2.0f * ZP.xx, 2.0f * ZP.yy, 2.0f * ZP.zz, 2.0f * ZP.xy, 2.0f * ZP.xz, 2.0f * ZP.yz};
+  L.s = s;
+  return true;
+}
+
+// tau = mu ln b + (lam/2) tr(ln b) I from ln b = ln(s) I + A.
+MPM_HD M3<float> tau_from_log(const S3& A, float s, float mu, float lam) {
+  const float ls = std::log(s);
+  const float d = mu * ls + 0.5f * lam * (3.0f * ls + A.xx + A.yy + A.zz);
+  M3<float> t = s3_full(S3{mu * A.xx + d, mu * A.yy + d, mu * A.zz + d, mu * A.xy, mu * A.xz,
+                           mu * A.yz});
+  return t;
+}
+
+// General x symmetric and symmetric x general products for the reverse pass.
+MPM_HD M3<float> mul_gs(const M3<float>& A, const S3& B) { return mul(A, s3_full(B)); }
+MPM_HD M3<float> mul_sg(const S3& A, const M3<float>& B) { return mul(s3_full(A), B); }
+
+// Fe cotangent through the series: reverse mode over log_series and
+// tau_from_log, given tau_bar (symmetrised) tb.
+MPM_HD M3<float> hencky_fe_bar_series(const M3<float>& Fe, const M3<float>& b, const M3<float>& tb,
+                                      float mu, float lam, const LogSeries& L) {
+  // tau = mu Lg + (lam/2) tr(Lg) I  ->  Lg_bar = M
+  M3<float> M = scaled(tb, mu);
+  const float hl = 0.5f * lam * trace(tb);
+  M.a[0] += hl; M.a[4] += hl; M.a[8] += hl;
+  // Lg = ln(s) I + A
+  float sbar = trace(M) / L.s;
+  // A = 2 Z P0
+  M3<float> Zb = scaled(mul_gs(M, L.P[0]), 2.0f);
+  M3<float> Pb = scaled(mul_sg(L.Z, M), 2.0f);
+  // P_k = Y P_{k+1} + I / (2k + 1), P_N = I / (2N + 1)
+  M3<float> Yb = zero3<float>();
+  for (int k = 0; k < kLogTerms; ++k) {
+    if (k + 1 < kLogTerms) {
+      const M3<float> t = mul_gs(Pb, L.P[k + 1]);
+      for (int i = 0; i < 9; ++i) Yb.a[i] += t.a[i];
+    } else {
+      const float aN = 1.0f / (2 * kLogTerms + 1);
+      for (int i = 0; i < 9; ++i) Yb.a[i] += aN * Pb.a[i];
+    }
+    Pb = mul_sg(L.Y, Pb);
+  }
+  // Y = Z Z
+  {
+    const M3<float> t1 = mul_gs(Yb, L.Z), t2 = mul_sg(L.Z, Yb);
+    for (int i = 0; i < 9; ++i) Zb.a[i] += t1.a[i] + t2.a[i];
+  }
+  // Z = E W with E = b / s - I;  W = (I + b / s)^{-1}
+  const float r = 1.0f / L.s;
+  const S3 E = {b.a[0] * r - 1.0f, b.a[4] * r - 1.0f, b.a[8] * r - 1.0f,
+                b.a[1] * r, b.a[2] * r, b.a[5] * r};
+  const M3<float> Eb = mul_gs(Zb, L.W);
+  const M3<float> Wb = mul_sg(E, Zb);
+  const M3<float> WWbW = mul_gs(mul_sg(L.W, Wb), L.W);
+  M3<float> bpb;  // cotangent of b' = b / s
+  for (int i = 0; i < 9; ++i) bpb.a[i] = Eb.a[i] - WWbW.a[i];
+  // b' = r b, r = 1 / s, s = tr b / 3
+  const float rbar = frob(bpb, b);
+  sbar -= rbar * r * r;
+  M3<float> bb = scaled(bpb, r);
+  const float d = sbar * (1.0f / 3.0f);
+  bb.a[0] += d; bb.a[4] += d; bb.a[8] += d;
+  // b = Fe Fe^T  ->  Fe_bar = (bb + bb^T) Fe
+  M3<float> S;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) S(i, j) = bb(i, j) + bb(j, i);
+  return mul(S, Fe);
+}
+
 MPM_HD M3<double> hencky_fe_bar(const M3<double>& Fe, const M3<double>& tb, double mu, double lam,
                                 M3<double>* tau_out = nullptr) {
   double beta[3];
@@ -267,6 +420,17 @@ MPM_HD M3<double> hencky_fe_bar(const M3<double>& Fe, const M3<double>& tb, doub
 }
 MPM_HD M3<float> hencky_fe_bar(const M3<float>& Fe, const M3<float>& tb, float mu, float lam,
                                M3<float>* tau_out = nullptr) {
+#if MPM_HENCKY_SERIES
+  {
+    const M3<float> b = mul_nt(Fe, Fe);
+    LogSeries L;
+    S3 A;
+    if (log_series(b, L, A)) {
+      if (tau_out) *tau_out = tau_from_log(A, L.s, mu, lam);
+      return hencky_fe_bar_series(Fe, b, tb, mu, lam, L);
+    }
+  }
+#endif
   float beta[3];
   M3<float> U;
   sym_eig3(mul_nt(Fe, Fe), beta, U);
@@ -290,6 +454,13 @@ MPM_HD M3<double> hencky_tau(const M3<double>& Fe, double mu, double lam) {
 }
 MPM_HD M3<float> hencky_tau(const M3<float>& Fe, float mu, float lam) {
   // tau is smooth through repeated eigenvalues: FP32 throughout.
+#if MPM_HENCKY_SERIES
+  {
+    LogSeries L;
+    S3 A;
+    if (log_series(mul_nt(Fe, Fe), L, A)) return tau_from_log(A, L.s, mu, lam);
+  }
+#endif
   return hencky_tau_core(Fe, mu, lam);
 }
 
