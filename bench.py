@@ -141,6 +141,8 @@ def parse():
                     help="for rocprofv3 --pmc: a device copy of --copy-mb MiB (counter "
                          "calibration), then one eager forward + adjoint iteration; no timing")
     ap.add_argument("--copy-mb", type=int, default=256)
+    ap.add_argument("--no-others", action="store_true",
+                    help="skip the single-scene workloads (bilayer strip, bioink blob, shell)")
     ap.add_argument("--inversion-iters", type=int, default=50,
                     help="Gauss-Newton iterations of the full inversion report (the timed "
                          "ones included); 0 reports none (profiling runs)")
@@ -241,6 +243,55 @@ def cpu_baseline(scenes, budget_s):
                       f"(FP64 vectorised numpy oracle, 1 thread per process, {P} processes, "
                       f"{done} particle-steps in {wall:.1f} s)",
             "single_thread_value": n0 / t0}
+
+
+def other_workloads():
+    """Forward + checkpointed adjoint particle-steps/s of the single-scene
+    workloads at full size, FP32, through autograd.GraphRollout (captured
+    once, then replayed; the replayed call is timed, inputs resident on the
+    device): the bilayer strip (65,536 particles, 500 steps, Hencky), the
+    bioink blob (2,097,152, 300 steps, Herschel-Bulkley, gradient w.r.t.
+    (tau_y, K)) and the control-grid shell (~0.91 M, 300 steps, Hencky)."""
+    import numpy as np
+    import torch
+    from morphompm_amd.autograd import GraphRollout
+    from morphompm_amd.scenes import bilayer_strip, bioink_blob, shell
+    out = {}
+    for name, build, K in (("bilayer_strip_65k_x500", bilayer_strip, 50),
+                           ("bioink_2.1M_x300", bioink_blob, 25),
+                           ("shell_0.9M_x300", shell, 25)):
+        sc = build()
+        n, dt = sc.n, torch.float32
+        x0 = torch.tensor(sc.pts, dtype=dt, device="cuda")
+        z3 = torch.zeros(n, 3, dtype=dt, device="cuda")
+        z33 = torch.zeros(n, 3, 3, dtype=dt, device="cuda")
+        F0 = torch.eye(3, dtype=dt, device="cuda").repeat(n, 1, 1)
+        G = torch.tensor(sc.G, dtype=dt, device="cuda")
+        hb = sc.model.kind == "hb"
+        run = GraphRollout(x0, sc.model, sc.cfg, sc.n_steps, True, checkpoint_every=K, hb=hb)
+        th = torch.tensor([sc.model.tau_y, sc.model.K], dtype=dt, device="cuda") if hb else None
+        ms = []
+        for rep in range(2):
+            Gr = G.clone().requires_grad_(True)
+            thr = th.clone().requires_grad_(True) if hb else None
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            x, v, C, F = run(x0, z3, z33, F0, Gr, thr)
+            L = (x.double() ** 2).sum() + torch.linalg.det(F.double()).sum()
+            L.backward()
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        out[name] = {"particles": n, "steps": sc.n_steps, "checkpoint_every": K,
+                     "model": sc.model.kind, "ms_fwd_adj": ms[-1],
+                     "particle_steps_per_s": n * sc.n_steps / (ms[-1] / 1e3),
+                     "timed": "second call (graph replay), forward + adjoint + loss"}
+        print(f"[bench] {name}: {out[name]['particle_steps_per_s']:.3g} particle-steps/s",
+              file=sys.stderr, flush=True)
+        del run, x0, G, F0, z3, z33
+        torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -496,6 +547,8 @@ def main():
         del b64
         torch.cuda.empty_cache()
 
+    others = None if a.no_others else other_workloads()
+
     cpu = None
     if not a.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
@@ -520,7 +573,8 @@ def main():
            "batched_over_plain": (value / plain["per_instance_value"]) if plain else None,
            "batched_over_cpu": (value / cpu["value"]) if cpu else None,
            "fp32_vs_fp64": acc,
-           "inversion": inversion}
+           "inversion": inversion,
+           "single_scene_workloads": others}
     print(json.dumps(out))
 
 
