@@ -302,7 +302,9 @@ int32_t mpm_gauss_newton(int32_t n_inst, const double* loss, const double* dloss
  * events[2i+1] just after it on its stream, for the i-th such launch while
  * i < n_pairs; launches under stream capture are not timed (event records
  * are refused inside a capture on this stack).  kernel_id < 0 disarms.
- * Process-wide state, not thread-safe.  mpm_kernel_timer_count() = launches
+ * The timer is per host thread: it times the launches issued by the thread
+ * that armed it, and calls from other threads neither see nor disturb it.
+ * mpm_kernel_timer_count() = launches
  * recorded. */
 enum {
   MPM_K_SORT = 0,        /* the whole counting-sort sequence */

@@ -52,11 +52,15 @@ def cofactor_rows(F):
 
 class BatchedInverse:
     def __init__(self, scenes, dtype=torch.float32, tile=DEFAULT_TILE, block=256,
-                 use_graphs=True, checkpoint_every=None, keep_segments=1):
+                 use_graphs=True, checkpoint_every=None, keep_segments=1,
+                 iterations_per_graph=1):
         """``tile`` / ``block``: the P2G tile shape and particle-kernel block
         size, or "auto" to pick them with the autotuner (cached per shape
         class).  ``keep_segments``: step-record stores the forward pass keeps
-        (see ``engine.Trajectory``); more trade HBM for less recompute."""
+        (see ``engine.Trajectory``); more trade HBM for less recompute.
+        ``iterations_per_graph``: Gauss-Newton iterations captured in one HIP
+        graph (``gn_iterate`` replays it for whole multiples and runs any
+        remainder as single-iteration replays)."""
         self.scenes = scenes
         steps = {s.n_steps for s in scenes}
         if len(steps) != 1:
@@ -107,7 +111,9 @@ class BatchedInverse:
         self.hist_len = torch.zeros(ni, **i32)
         self.tol = 1e-9
         self.use_graphs = use_graphs
-        self._g_fwd = self._g_bwd = self._g_iter = None
+        self.iterations_per_graph = max(1, int(iterations_per_graph))
+        self._g_fwd = self._g_bwd = None
+        self._g_iter = {}        # iterations per graph -> captured graph
         self._g_iter_key = None
         self._final = None
 
@@ -185,10 +191,10 @@ class BatchedInverse:
             self._grad()
         self._seg_after_bwd = self.traj.store_state()
 
-    def _capture_iteration(self):
-        """One whole Gauss-Newton iteration as one graph: growth, forward,
-        loss, reverse, gradient, update.  The warm-up runs the update on a
-        copy of the inversion state, so capturing does not advance it."""
+    def _capture_iteration(self, k):
+        """k whole Gauss-Newton iterations as one graph: growth, forward,
+        loss, reverse, gradient, update, k times.  The warm-up runs the update
+        on a copy of the inversion state, so capturing does not advance it."""
         def phases(state):
             self._growth()
             self._fwd()
@@ -201,11 +207,12 @@ class BatchedInverse:
         theta0 = self.theta.clone()
         self._warm(lambda: phases(scratch))
         self.theta.copy_(theta0)
-        self._g_iter = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self._g_iter):
-            phases(None)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(k):
+                phases(None)
+        self._g_iter[k] = g
         self._seg_after_iter = self.traj.store_state()
-        self._g_iter_key = (self.tol, self.hist.data_ptr(), self.hist.shape[1])
 
     def run_forward(self):
         """Forward pass and loss of the current growth tensors."""
@@ -254,29 +261,44 @@ class BatchedInverse:
         self.done.zero_()
         self.its.zero_()
 
+    def _graph(self, k):
+        key = (self.tol, self.hist.data_ptr(), self.hist.shape[1])
+        if self._g_iter_key != key:          # the update's arguments changed
+            self._g_iter = {}
+            self._g_iter_key = key
+        if k not in self._g_iter:
+            self._capture_iteration(k)
+        return self._g_iter[k]
+
     def prepare_iteration(self):
-        """Captures the iteration graph now (when use_graphs), so that later
-        gn_iterate calls only replay it."""
-        if self.use_graphs and self._g_iter is None:
-            self._capture_iteration()
+        """Captures the iteration graphs now (when use_graphs), so that later
+        gn_iterate calls only replay them."""
+        if self.use_graphs:
+            self._graph(self.iterations_per_graph)
+            if self.iterations_per_graph > 1:
+                self._graph(1)
 
     def gn_iterate(self, n=1):
         """n Gauss-Newton iterations of every unconverged instance, on the
-        device (graph replays when use_graphs); no host synchronisation."""
-        for _ in range(n):
-            if self.use_graphs:
-                key = (self.tol, self.hist.data_ptr(), self.hist.shape[1])
-                if self._g_iter is None or self._g_iter_key != key:
-                    self._capture_iteration()
-                self._g_iter.replay()
-                self.traj.set_store_state(self._seg_after_iter)
-            else:
+        device (graph replays when use_graphs: n // iterations_per_graph
+        replays of the multi-iteration graph, then single-iteration ones);
+        no host synchronisation."""
+        if not self.use_graphs:
+            for _ in range(n):
                 self._growth()
                 self._fwd()
                 self._loss()
                 self._bwd()
                 self._grad()
                 self._update()
+            return
+        k = self.iterations_per_graph
+        for _ in range(n // k):
+            self._graph(k).replay()
+            self.traj.set_store_state(self._seg_after_iter)
+        for _ in range(n % k):
+            self._graph(1).replay()
+            self.traj.set_store_state(self._seg_after_iter)
 
     def infer(self, targets, theta0, iters=50, tol=1e-9, return_history=False):
         """Reference Gauss-Newton per instance (diff.py:9-24): all instances are

@@ -1524,7 +1524,9 @@ struct KernelTimer {
   int n_pairs = 0;
   int count = 0;
 };
-KernelTimer g_timer;
+// Per host thread: arming the timer times the launches the same thread
+// issues, and other threads' calls neither see nor disturb it.
+thread_local KernelTimer g_timer;
 
 // Launches under stream capture are not timed: this stack refuses event
 // records (external or not) inside a capture.

@@ -247,6 +247,43 @@ def test_kernel_timer_eager_and_under_capture():
         L.mpm_kernel_timer(-1, None, 0)
 
 
+def test_kernel_timer_is_per_host_thread():
+    """An armed timer times the launches of the thread that armed it only:
+    an iteration run from another host thread records nothing in it."""
+    import ctypes
+    import threading
+    from morphompm_amd import _lib
+    from morphompm_amd.batched import BatchedInverse
+    L = _lib.lib()
+    sc = _small_batch(("nh", "hencky"), n_steps=6)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(64)]
+    for e in evs:
+        e.record()
+    torch.cuda.synchronize()
+    h = (ctypes.c_void_p * 64)(*[e.cuda_event for e in evs])
+    bi = BatchedInverse(sc, dtype=torch.float32, use_graphs=False)
+    bi.set_theta([s.extra["rate"] for s in sc])
+    bi.loss_and_grad()                       # first launches outside the timed part
+    torch.cuda.synchronize()
+    assert L.mpm_kernel_timer(4, h, 32) == 0
+    try:
+        seen = {}
+
+        def other():
+            bi.loss_and_grad()
+            torch.cuda.synchronize()
+            seen["count"] = L.mpm_kernel_timer_count()
+        t = threading.Thread(target=other)
+        t.start()
+        t.join()
+        assert seen["count"] == 0 and L.mpm_kernel_timer_count() == 0
+        bi.loss_and_grad()
+        torch.cuda.synchronize()
+        assert L.mpm_kernel_timer_count() > 0
+    finally:
+        L.mpm_kernel_timer(-1, None, 0)
+
+
 def test_kept_segments_under_graphs_match_eager():
     """Every segment kept (no recompute) with the forward and reverse passes
     captured as HIP graphs: losses, gradients and final states equal the
@@ -330,3 +367,25 @@ def test_device_update_matches_host_rule():
         assert hl[i].item() == (2 if app else 1), i
         if app:
             assert hist[i, 1].item() == g, i
+
+
+def test_multi_iteration_graph_matches_eager_device_loop():
+    """Several Gauss-Newton iterations captured in one HIP graph
+    (iterations_per_graph = 3): seven iterations (two replays of the
+    3-iteration graph, then one single-iteration replay) leave the eager
+    device loop's rates, flags, counts and histories, bit for bit."""
+    from morphompm_amd.batched import BatchedInverse
+    sc = _small_batch(("hencky", "nh"), n_steps=8)
+    rates = np.array([s.extra["rate"] for s in sc])
+    eager = BatchedInverse(sc, dtype=torch.float32, use_graphs=False)
+    multi = BatchedInverse(sc, dtype=torch.float32, use_graphs=True, iterations_per_graph=3)
+    eager.set_theta(rates)
+    target = eager.loss_and_grad()[0].cpu().numpy()
+    for bi in (eager, multi):
+        bi.start_inversion(target, 0.5 * rates, tol=1e-9, hist_cap=9)
+        bi.gn_iterate(7)
+    torch.cuda.synchronize()
+    for a, b in ((eager.theta, multi.theta), (eager.done, multi.done), (eager.its, multi.its),
+                 (eager.hist, multi.hist), (eager.hist_len, multi.hist_len)):
+        assert torch.equal(a, b)
+    assert int(multi.its.max()) >= 3
