@@ -65,6 +65,35 @@ def test_hencky_degenerate_golden(golden):
         assert np.all(np.isfinite(Fb32[i]))
 
 
+def test_fp32_hencky_series_on_device_matches_fp64():
+    """The device's FP32 Hencky path (the matrix-log series while |Z|_F <=
+    0.2, the FP32 eigen path beyond) against its FP64 eigen path, on the
+    deformation families of the host-build test: exactly isotropic (repeated
+    eigenvalues), then log-stretch spreads 1e-7 .. 0.25 that end past the
+    series' range, with the stretch ratios of the last family straddling the
+    switch.  Bounds as on the host: stress within 3e-6, F and Fg cotangents
+    within 2e-6 of the largest FP64 entry."""
+    m = _gpu_models()["hencky"]
+    rng = np.random.default_rng(0)
+    for sd in (0.0, 1e-7, 1e-4, 1e-2, 0.05, 0.1, 0.25):
+        n = 400
+        R = np.linalg.qr(rng.standard_normal((n, 3, 3)))[0]
+        st = np.exp(sd * rng.standard_normal((n, 3)))
+        F = R @ (st[:, :, None] * np.linalg.qr(rng.standard_normal((n, 3, 3)))[0])
+        if sd == 0.0:
+            F = np.repeat(np.eye(3)[None], n, 0) * 1.03
+        Fg = np.repeat(np.eye(3)[None], n, 0) * rng.uniform(0.9, 1.1, (n, 1, 1))
+        tb = rng.standard_normal((n, 3, 3))
+        tb = tb + tb.transpose(0, 2, 1)
+        f32 = lambda a: a.astype(np.float32)
+        tau64 = m.stress(F, Fg, None)
+        tau32 = m.stress(f32(F), f32(Fg), None)
+        assert tau32.dtype == np.float32 and _rel(tau32, tau64) < 3e-6, sd
+        Fb64, Fgb64, _ = m.stress_vjp(F, Fg, None, tb)
+        Fb32, Fgb32, _ = m.stress_vjp(f32(F), f32(Fg), None, f32(tb))
+        assert _rel(Fb32, Fb64) < 2e-6 and _rel(Fgb32, Fgb64) < 2e-6, sd
+
+
 def test_hb_param_vjp_matches_oracle():
     m = _gpu_models()["hb"]
     rng = np.random.default_rng(5)
