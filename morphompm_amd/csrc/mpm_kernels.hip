@@ -1022,12 +1022,15 @@ __global__ __launch_bounds__(256) void k_g2p(const mpm_instance* inst, const int
       for (int i = 0; i < 3; ++i) {
         const T wxi = i == 0 ? w[0][0] : (i == 1 ? w[0][1] : w[0][2]);
         const T Ti = T(i);
-#pragma unroll
+        // z-rows stay rolled too (77-79 VGPRs in FP32, occupancy 6); the
+        // row's y weight is picked by select
+#pragma unroll 1
         for (int jj = 0; jj < 3; ++jj) {
           const V4<T> g0 = nodes(i, jj, 0), g1 = nodes(i, jj, 1), g2 = nodes(i, jj, 2);
           const T v0[3] = {g0.y, g0.z, g0.w}, v1[3] = {g1.y, g1.z, g1.w},
                   v2[3] = {g2.y, g2.z, g2.w};
-          const T wxy = wxi * w[1][jj];
+          const T wyj = jj == 0 ? w[1][0] : (jj == 1 ? w[1][1] : w[1][2]);
+          const T wxy = wxi * wyj;
           for (int c = 0; c < 3; ++c) {
             const T r0 = wz0 * v0[c] + wz1 * v1[c] + wz2 * v2[c];
             const T r1 = wz1 * v1[c] + wz2k * v2[c];
