@@ -62,7 +62,7 @@ def _rate_loss_grad(sc, rate, dtype, loss_fn, n_steps=None, K=None):
                             checkpoint_every=K or sc.checkpoint_every, sched=sched)
     L = loss_fn(x, F)
     L.backward()
-    return float(L), float(r.grad)
+    return float(L.detach()), float(r.grad)
 
 
 # ------------------------------------------------------------------ free swell
