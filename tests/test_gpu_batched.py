@@ -50,12 +50,12 @@ def _oracle_instance(s, rate):
     m = _oracle_model(s.model)
     st0 = RL.ParticleState.rest(s.pts)
     st0.Fg = np.eye(3) + rate * s.dG
-    fin, caches = RV.rollout(st0, m, cfg, s.n_steps, advect=True, sched=s.sched)
+    fin, caches = RV.rollout(st0, m, cfg, s.n_steps, advect=s.advect, sched=s.sched)
     det = np.linalg.det(fin.F)
     n = s.n
     Fb = det[:, None, None] * np.swapaxes(np.linalg.inv(fin.F), 1, 2)
     z3, z33 = np.zeros((n, 3)), np.zeros((n, 3, 3))
-    Gb = RV.rollout_vjp(caches, z3, z3, z33, Fb, z33, advect=True, sched=s.sched)[4]
+    Gb = RV.rollout_vjp(caches, z3, z3, z33, Fb, z33, advect=s.advect, sched=s.sched)[4]
     return fin, det.sum(), float(np.sum(Gb * s.dG))
 
 
@@ -71,6 +71,25 @@ def test_batched_fp64_matches_oracle(models):
         x, v, C, F = (t.cpu().numpy() for t in p.unpack_dyn(bi.final_state(), i, bi.perm))
         assert np.max(np.abs(F - fin.F)) <= 1e-10 * np.max(np.abs(fin.F)), i
         assert np.max(np.abs(x - fin.x)) <= 1e-12, i
+        assert abs(L[i] - Lo) <= 1e-12 * abs(Lo), i
+        assert abs(g[i] - go) <= 1e-9 * abs(go), (i, g[i], go)
+
+
+def test_batched_without_advection_matches_oracle_fp64():
+    """advect=False (positions fixed, transfer.py:75-77; x_bar = 0) over a
+    whole batch: FP64 loss and rate gradient against the oracle, and the
+    particles stay where they started."""
+    sc = _small_batch(("nh", "hencky", "hb"), n_steps=10)
+    for s in sc:
+        s.advect = False
+    bi, L, g = _run_batch(sc, torch.float64)
+    assert bi.advect is False
+    p = bi.plan
+    for i, s in enumerate(sc):
+        fin, Lo, go = _oracle_instance(s, s.extra["rate"])
+        x, v, C, F = (t.cpu().numpy() for t in p.unpack_dyn(bi.final_state(), i, bi.perm))
+        assert np.array_equal(x, s.pts) or np.max(np.abs(x - s.pts)) <= 1e-15, i
+        assert np.max(np.abs(F - fin.F)) <= 1e-10 * np.max(np.abs(fin.F)), i
         assert abs(L[i] - Lo) <= 1e-12 * abs(Lo), i
         assert abs(g[i] - go) <= 1e-9 * abs(go), (i, g[i], go)
 
