@@ -635,11 +635,16 @@ __global__ __launch_bounds__(256) void k_cell_keys(const mpm_instance* inst,
 }
 
 // ---------------------------------------------------------------------------
-// P2G as a cell-binned segmented reduction into an LDS node tile.
+// P2G as a cell-binned segmented reduction into LDS node tiles.
 // Tile = TX*TY*TZ bins (contiguous keys), one thread each; node tile
-// (TX+2)(TY+2)(TZ+2).  One block per occupied unit of this step (blocks past
-// n_active and dead units return at once).  partial: [active unit][TN] node
-// records (sum of weights (MASS), sum of w (Q + A' o)), one quad per node.
+// (TX+2)(TY+2)(TZ+2), one per wave.  One block per occupied unit of this step
+// (blocks past n_active and dead units return at once).  Each thread sums its
+// bin's particles in sorted order, its wave adds the bins into the wave's
+// node tile in 27 rounds (one stencil node per round), and the last wave to
+// finish sums the waves' tiles in wave order: a fixed order throughout, no
+// float atomics, no block barrier between rounds.  partial: [active unit][TN]
+// node records (sum of weights (MASS), sum of w (Q + A' o)), one quad per
+// node.
 // ---------------------------------------------------------------------------
 // INDIRECT: payload records in slot order, read through sorted[j]; else in
 // sorted order.
@@ -654,7 +659,10 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
                                                         const T* payload, T* partial) {
   constexpr int NY = TY + 2, NZ = TZ + 2, TN = (TX + 2) * NY * NZ;
   constexpr int NT = TX * TY * TZ;
-  __shared__ V4<T> lds[TN];
+  constexpr int NW = NT / 64;  // waves; each has its own node tile
+  static_assert(NT % 64 == 0, "whole waves");
+  __shared__ V4<T> ldsw[NW][TN];
+  __shared__ int waves_done;
   const int64_t a = blockIdx.x;
   if (a >= *n_active) return;
   const int32_t u = active_unit[a];
@@ -664,7 +672,10 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
   const int tid = threadIdx.x;
   const int lz = tid % TZ, ly = (tid / TZ) % TY, lx = tid / (TY * TZ);
 
-  for (int i = tid; i < TN; i += NT) lds[i] = V4<T>{T(0), T(0), T(0), T(0)};
+  const int lane = tid & 63, wave = tid >> 6;
+  V4<T>* lds = ldsw[wave];
+  for (int i = lane; i < TN; i += 64) lds[i] = V4<T>{T(0), T(0), T(0), T(0)};
+  if (tid == 0) waves_done = 0;
   // INDIRECT: the tile's particles are the contiguous sorted range [r0, r1);
   // their slot indices are staged in LDS with coalesced loads, so the record
   // loads do not wait on a dependent index load each.
@@ -673,8 +684,8 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
   if (INDIRECT) {
     const int32_t n_idx = min(bin_start[key0 + NT] - r0, kTileIdxCap);
     for (int i = tid; i < n_idx; i += NT) sidx[i] = sorted[r0 + i];
-    __syncthreads();
   }
+  __syncthreads();
 
   T aw[27], am[27][3];
 #pragma unroll
@@ -714,9 +725,14 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
         }
       }
   }
+  // Each wave adds its bins into its own node tile, 27 rounds with no block
+  // barrier: a wave's LDS accesses complete in order, and the fences keep the
+  // compiler from moving one round's accesses across another's.
 #pragma unroll
   for (int o = 0; o < 27; ++o) {
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int nl = ((lx + o / 9) * NY + (ly + (o / 3) % 3)) * NZ + (lz + o % 3);
     V4<T> t = lds[nl];
     if (MASS) t.x += aw[o];
@@ -725,10 +741,26 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_p2g_tile(const mpm_instance* ins
     t.w += am[o][2];
     lds[nl] = t;
   }
-  __syncthreads();
+  // The last wave to finish sums the waves' tiles in wave order and writes
+  // the partial; the others end here.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  int prev = 0;
+  if (lane == 0) prev = __hip_atomic_fetch_add(&waves_done, 1, __ATOMIC_ACQ_REL,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+  prev = __shfl(prev, 0);
+  if (prev != NW - 1) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   T* out = partial + a * (int64_t)(4 * TN);
-  for (int i = tid; i < TN; i += NT) {
-    const V4<T> t = lds[i];
+  for (int i = lane; i < TN; i += 64) {
+    V4<T> t = ldsw[0][i];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+      const V4<T> s2 = ldsw[w][i];
+      t.x += s2.x;
+      t.y += s2.y;
+      t.z += s2.z;
+      t.w += s2.w;
+    }
     st4(out + 4 * i, t.x, t.y, t.z, t.w);
   }
 }

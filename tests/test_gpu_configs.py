@@ -277,7 +277,7 @@ def test_bioink_reduced_fp64_oracle_and_theta():
                                 hb_params=th)
         L = (x.double() ** 2).sum() + torch.linalg.det(F.double()).sum()
         L.backward()
-        return float(L), th.grad.double().cpu().numpy()
+        return float(L.detach()), th.grad.double().cpu().numpy()
     L0, g64 = run([50.0, 10.0], torch.float64)
     for k, h in ((0, 1e-3), (1, 1e-4)):
         tp = [50.0, 10.0]; tm = [50.0, 10.0]
@@ -309,7 +309,7 @@ def test_shell_control_grid_gradient_fd_directions():
         L = position_misfit(x.double(), target)
         if grad:
             L.backward()
-            return float(L), th.grad.double()
+            return float(L.detach()), th.grad.double()
         return float(L), None
     _, g64 = run(theta0, torch.float64)
     rng = np.random.default_rng(0)
@@ -457,7 +457,7 @@ def test_shell_full_size_control_grid_fd32():
             L = position_misfit(x.double(), target)
         if grad:
             L.backward()
-            return float(L), th.grad.double()
+            return float(L.detach()), th.grad.double()
         return float(L), None
     (_, g64), t64 = _timed(lambda: run(theta0, torch.float64))
     print(f"shell FP64 gradient in {t64:.2f} s", flush=True)
