@@ -8,9 +8,10 @@
 //              only, for counts and places in a bin) -> per-bin sort by
 //              slot, so the order inside every bin is fixed;
 //   p2g tile : one thread per bin sums its particles' 27 stencil contributions
-//              in registers, then adds them into an LDS node tile in 27
-//              barrier-separated rounds (one writer per node per round): a
-//              cell-binned segmented reduction with no float atomics;
+//              in registers, then adds them into its wave's LDS node tile in
+//              27 rounds (one writer per node per round); the waves' tiles
+//              are summed in wave order: a cell-binned segmented reduction
+//              with no float atomics;
 //   grid     : each node sums the <= 8 tile partials covering it in a fixed
 //              order, then applies v = (1 - damping) mom / m;
 //   g2p      : 27-node gather, new v, C, x, F = (I + dt C) F.
