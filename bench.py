@@ -246,20 +246,24 @@ def cpu_baseline(scenes, budget_s):
 
 
 def other_workloads():
-    """Forward + checkpointed adjoint particle-steps/s of the single-scene
-    workloads at full size, FP32, through autograd.GraphRollout (captured
-    once, then replayed; the replayed call is timed, inputs resident on the
-    device): the bilayer strip (65,536 particles, 500 steps, Hencky), the
-    bioink blob (2,097,152, 300 steps, Herschel-Bulkley, gradient w.r.t.
-    (tau_y, K)) and the control-grid shell (~0.91 M, 300 steps, Hencky)."""
+    """Forward + adjoint particle-steps/s of the single-scene workloads at full
+    size, FP32, through autograd.GraphRollout (captured once, then replayed;
+    the replayed call is timed, inputs resident on the device): the bilayer
+    strip (65,536 particles, 500 steps, Hencky), the bioink blob (2,097,152,
+    300 steps, Herschel-Bulkley, gradient w.r.t. (tau_y, K)) and the
+    control-grid shell (~0.91 M, 300 steps, Hencky).  The bioink adjoint
+    checkpoints every 25 steps and recomputes in between, as that workload
+    specifies; the bilayer and the shell specify no checkpointing, so their
+    forward pass keeps every step record that fits in HBM
+    (keep_segments="auto") and the adjoint recomputes only the rest."""
     import numpy as np
     import torch
     from morphompm_amd.autograd import GraphRollout
     from morphompm_amd.scenes import bilayer_strip, bioink_blob, shell
     out = {}
-    for name, build, K in (("bilayer_strip_65k_x500", bilayer_strip, 50),
-                           ("bioink_2.1M_x300", bioink_blob, 25),
-                           ("shell_0.9M_x300", shell, 25)):
+    for name, build, K, keep in (("bilayer_strip_65k_x500", bilayer_strip, 50, "auto"),
+                                 ("bioink_2.1M_x300", bioink_blob, 25, 1),
+                                 ("shell_0.9M_x300", shell, 25, "auto")):
         sc = build()
         n, dt = sc.n, torch.float32
         x0 = torch.tensor(sc.pts, dtype=dt, device="cuda")
@@ -268,7 +272,8 @@ def other_workloads():
         F0 = torch.eye(3, dtype=dt, device="cuda").repeat(n, 1, 1)
         G = torch.tensor(sc.G, dtype=dt, device="cuda")
         hb = sc.model.kind == "hb"
-        run = GraphRollout(x0, sc.model, sc.cfg, sc.n_steps, True, checkpoint_every=K, hb=hb)
+        run = GraphRollout(x0, sc.model, sc.cfg, sc.n_steps, True, checkpoint_every=K, hb=hb,
+                           keep_segments=keep)
         th = torch.tensor([sc.model.tau_y, sc.model.K], dtype=dt, device="cuda") if hb else None
         ms = []
         for rep in range(2):
@@ -284,10 +289,13 @@ def other_workloads():
             torch.cuda.synchronize()
             ms.append(e0.elapsed_time(e1))
         out[name] = {"particles": n, "steps": sc.n_steps, "checkpoint_every": K,
+                     "segments_kept": run.traj.M,
+                     "segments": -(-sc.n_steps // K),
                      "model": sc.model.kind, "ms_fwd_adj": ms[-1],
                      "particle_steps_per_s": n * sc.n_steps / (ms[-1] / 1e3),
                      "timed": "second call (graph replay), forward + adjoint + loss"}
-        print(f"[bench] {name}: {out[name]['particle_steps_per_s']:.3g} particle-steps/s",
+        print(f"[bench] {name}: {out[name]['particle_steps_per_s']:.3g} particle-steps/s "
+              f"({run.traj.M} of {out[name]['segments']} segments kept)",
               file=sys.stderr, flush=True)
         del run, x0, G, F0, z3, z33
         torch.cuda.empty_cache()
