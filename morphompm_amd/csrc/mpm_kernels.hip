@@ -1185,9 +1185,11 @@ __global__ __launch_bounds__(256) void k_grid_adj(const mpm_instance* inst, cons
 }
 
 // reverse step, part 3 (transfer.py:100-119 x_bar terms, :129-156), one
-// thread per particle slot p (coalesced state rows), at sorted position
-// j = rank[p]; each 256-slot chunk stages its stencil box of the grid adjoint
-// (and, with advection, of the grid) in LDS.
+// thread per particle slot p (coalesced state rows, read and written once,
+// so non-temporal: they leave the caches to the staged boxes and the
+// gathered payloads), at sorted position j = rank[p]; each 256-slot chunk
+// stages its stencil box of the grid adjoint (and, with advection, of the
+// grid) in LDS.
 //
 // The P2G-adjoint gather reads the forward payload pf (fx, q2, A', cell) of p
 // and the adjoint payload pa (fx, q1, P1) at j.  With dpos = (o - fx) dx for
@@ -1326,7 +1328,7 @@ __global__ __launch_bounds__(256) void k_adj_particle(
       if (advect) {
         // x_bar += xn_b - (4 inv_dx^2 C_b')^T (sum w v_i) - A^T (sum w mb_i), A = A'/dx
         T xnb[3];
-        ld3(bar_out, n_slots, p, RX, xnb);
+        ld3_nt(bar_out, n_slots, p, RX, xnb);
         for (int c = 0; c < 3; ++c)
           xb[c] += xnb[c] - ((P1(0, c) * Sv[0] + P1(1, c) * Sv[1] + P1(2, c) * Sv[2]) +
                              (Ap(0, c) * vb[0] + Ap(1, c) * vb[1] + Ap(2, c) * vb[2])) * I.inv_dx;
@@ -1335,44 +1337,44 @@ __global__ __launch_bounds__(256) void k_adj_particle(
         for (int e = 0; e < 3; ++e) MD(c, e) = I.dx * (Mo(c, e) - vb[c] * fx[e]);
       for (int c = 0; c < 3; ++c) vb[c] *= I.pmass;
     }
-    st3(bar_in, n_slots, p, RX, xb);
-    st3(bar_in, n_slots, p, RV, vb);
+    st3_nt(bar_in, n_slots, p, RX, xb);
+    st3_nt(bar_in, n_slots, p, RV, vb);
 
     const T a_t = sched ? sched[ii] : T(1);
-    const M3<T> Fg = grown(ld9(Fgp, n_slots, p, 0), a_t);
+    const M3<T> Fg = grown(ld9_nt(Fgp, n_slots, p, 0), a_t);
     const T detG = det(Fg);
     const T c = -I.k4 * I.vol0 * detG;
     const M3<T> skb = scaled(MD, I.dt);
-    const M3<T> C = ld9(dyn_in, n_slots, p, RC);
+    const M3<T> C = ld9_nt(dyn_in, n_slots, p, RC);
     M3<T> Fpb, Fgb, Cbar, tau;
     {
-      const M3<T> F = ld9(dyn_in, n_slots, p, RF);
+      const M3<T> F = ld9_nt(dyn_in, n_slots, p, RF);
       kirchhoff_vjp(I.mp, F, Fg, C, scaled(skb, c), Fpb, Fgb, Cbar, th, &tau);
     }
     M3<T> Cbo = scaled(MD, I.pmass);
     for (int k = 0; k < 9; ++k) Cbo.a[k] += Cbar.a[k];
     // with PREV, C_bar goes into the previous step's payload below and no
     // later kernel of the chain reads it from bar_in
-    if (!PREV) st9(bar_in, n_slots, p, RC, Cbo);
+    if (!PREV) st9_nt(bar_in, n_slots, p, RC, Cbo);
     {
-      const M3<T> Cn = ld9(dyn_out, n_slots, p, RC);
-      const M3<T> Fnb = ld9(bar_out, n_slots, p, RF);
+      const M3<T> Cn = ld9_nt(dyn_out, n_slots, p, RC);
+      const M3<T> Fnb = ld9_nt(bar_out, n_slots, p, RF);
       // (I + dt C_new)^T Fn_b, formed as Fn_b + dt C_new^T Fn_b
       const M3<T> CtF = mul_tn(Cn, Fnb);
       for (int k = 0; k < 9; ++k) Fpb.a[k] += Fnb.a[k] + I.dt * CtF.a[k];
-      st9(bar_in, n_slots, p, RF, Fpb);
+      st9_nt(bar_in, n_slots, p, RF, Fpb);
     }
     if (PREV) {
       T xp[3];
-      ld3(dyn_prev, n_slots, p, RX, xp);
-      make_adj_payload(I, xp, ld9(dyn_prev, n_slots, p, RF), xb, vb, Cbo, Fpb, advect,
+      ld3_nt(dyn_prev, n_slots, p, RX, xp);
+      make_adj_payload(I, xp, ld9_nt(dyn_prev, n_slots, p, RF), xb, vb, Cbo, Fpb, advect,
                        payload_prev + (int64_t)rank_prev[p] * kRecord);
     }
     const M3<T> GiT = inv_t(Fg);
     const T vol = frob(skb, tau) * (-I.k4 * I.vol0) * detG;
-    M3<T> Gb = ld9(Fg_bar, n_slots, p, 0);
+    M3<T> Gb = ld9_nt(Fg_bar, n_slots, p, 0);
     for (int k = 0; k < 9; ++k) Gb.a[k] += a_t * (Fgb.a[k] + vol * GiT.a[k]);
-    st9(Fg_bar, n_slots, p, 0, Gb);
+    st9_nt(Fg_bar, n_slots, p, 0, Gb);
   }
   if (theta_acc != nullptr) {  // fixed-order block reduction, one slot per 256 particles
     __shared__ double tred[2][256];
