@@ -232,6 +232,56 @@ def test_batched_instances_at_real_size_match_oracle():
     assert np.all(np.abs(g32 - g64) <= 1e-5 * np.abs(g64))
 
 
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-6)])
+def test_batched_full_size_grid_conserves_mass_and_momentum(dtype, tol):
+    """One step of all 256 instances of the batched workload (2,667,384
+    particles) from stress-free states (F = Fg, so no internal forces) with
+    random velocities.  Per instance, from the grid record alone:
+      - the grid holds every particle's mass (the B-spline weights sum to 1,
+        transfer.py:43-45): rel. error <= tol;
+      - each node velocity over (1 - damping) is a mass-weighted mean of its
+        particles' velocities (transfer.py:53-58), so it lies within their
+        range (+- tol of the range);
+      - the grid momentum is (1 - damping) times the particles', less what
+        nodes at or below the 1e-12 mass threshold drop (at most their mass
+        times the fastest particle's speed), within tol of the momentum
+        scale."""
+    from morphompm_amd.batched import BatchedInverse
+    from morphompm_amd.scenes import batch_instances
+    sc = batch_instances(count=256, n_steps=200)
+    bi = BatchedInverse(sc, dtype=dtype, use_graphs=False)
+    rates = [s.extra["rate"] for s in sc]
+    bi.set_theta(rates)
+    p = bi.plan
+    rng = np.random.default_rng(11)
+    parts, vs = [], []
+    for s, r in zip(sc, rates):
+        v = rng.normal(0.0, 0.05, (s.n, 3))
+        Fg = np.eye(3)[None] + r * s.dG                # F = Fg: stress-free
+        parts.append((s.pts, v, np.zeros((s.n, 3, 3)), Fg))
+        vs.append(v)
+    dyn = p.pack_dyn(parts, dtype, bi.perm)
+    out, grid = p.new_dyn(dtype), p.new_grid(dtype)
+    p.step_fwd(dyn, bi.G, None, out, grid, p.new_sort(), p.new_payload(dtype), True)
+    g = grid.double().cpu().numpy()
+    worst = {"mass": 0.0, "range": 0.0, "momentum": 0.0}
+    for i, s in enumerate(sc):
+        gi = g[p.g_off[i]:p.g_off[i] + int(np.prod(s.cfg.dims))]
+        pm, keep = s.cfg.pmass, 1.0 - s.cfg.damping
+        m, v = gi[:, 0], gi[:, 1:]
+        worst["mass"] = max(worst["mass"], abs(m.sum() / (pm * s.n) - 1.0))
+        live = m > 1e-12
+        lo, hi = vs[i].min(0), vs[i].max(0)
+        over = np.maximum(lo - v[live] / keep, v[live] / keep - hi).max() / (hi - lo).max()
+        worst["range"] = max(worst["range"], float(over))
+        dropped = m[(m > 0) & ~live].sum() * np.abs(vs[i]).max()
+        scale = pm * np.abs(vs[i]).sum()
+        err = np.abs((m[:, None] * v).sum(0) - keep * pm * vs[i].sum(0)).max()
+        worst["momentum"] = max(worst["momentum"], float(max(0.0, err - keep * dropped) / scale))
+    print(f"{dtype}: " + ", ".join(f"{k} {w:.2e}" for k, w in worst.items()), flush=True)
+    assert worst["mass"] <= tol and worst["range"] <= tol and worst["momentum"] <= tol, worst
+
+
 @pytest.mark.parametrize("name", ["shell", "bioink"])
 def test_full_size_scene_fp64_matches_oracle(name):
     """The full control-grid shell (913,536 particles, 100^3, Hencky, trilinear
